@@ -1,0 +1,515 @@
+// C ABI of libqpesim.so (include/qpesim.h): handles, argument checks, error
+// mapping and the device-side StateVector.  No exceptions cross the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qpesim.h"
+#include "combinadic.hpp"
+#include "engine.hpp"
+#include "hamiltonian.hpp"
+#include "plan.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return QPE_OK;
+    } catch (const ApiError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return QPE_ERR_OOM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return QPE_ERR_HIP;
+    }
+}
+
+void require(bool ok, int code, const char *msg) {
+    if (!ok) throw ApiError(code, msg);
+}
+
+uint64_t choose(int n, int k) {
+    uint32_t tab[qpe::kBinomRows * qpe::kBinomCols];
+    qpe::fill_binomials(tab);
+    return qpe::binom(tab, n, k);
+}
+
+}  // namespace
+
+struct qpe_hamiltonian {
+    qpe::Hamiltonian h;
+    uint64_t version = 0;
+};
+
+struct qpe_state {
+    unsigned m = 0, n = 0, e = 0;
+    int dtype = QPE_COMPLEX128;
+    uint32_t C = 0;
+    uint64_t D = 0, M = 0;
+    size_t elem = 16;
+    void *amp = nullptr;
+    double *norms = nullptr, *scale = nullptr, *pk_partial = nullptr, *twiddles = nullptr, *probs = nullptr;
+    double *norm_partials = nullptr;
+    int *status = nullptr;
+    hipStream_t stream = nullptr;
+    int renorm = 0;
+    int engine = 0;
+    int64_t max_tile_bytes = 0;
+    bool scale_pending = false;
+    bool pk_valid = false;
+    int pk_grid = 0;
+    double t_evolve = 0, t_qift = 0, t_pk = 0;
+    int passes = 0, launches = 0;
+    std::unique_ptr<qpe::DevicePlan> plan;
+    const qpe_hamiltonian *plan_h = nullptr;
+    uint64_t plan_version = ~0ull;
+    int max_lds = 0;
+
+    ~qpe_state() {
+        plan.reset();
+        void *ptrs[] = {amp, norms, scale, pk_partial, twiddles, probs, norm_partials, status};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+constexpr int kNormParts = 512;
+constexpr int kPkGridMax = 1024;
+
+void check_state(qpe_state_t s) { require(s != nullptr, QPE_ERR_ARG, "null state handle"); }
+void check_ham(qpe_hamiltonian_t h) { require(h != nullptr, QPE_ERR_ARG, "null hamiltonian handle"); }
+
+void set_ones(double *dev, uint64_t count, hipStream_t st) {
+    std::vector<double> ones(count, 1.0);
+    qpe::hip_check(hipMemcpyAsync(dev, ones.data(), count * sizeof(double), hipMemcpyHostToDevice, st), "copy ones");
+    qpe::hip_check(hipStreamSynchronize(st), "sync");
+}
+
+// fold pending renormalisation factors into the stored amplitudes
+void materialize_scale(qpe_state_t s) {
+    if (!s->scale_pending) return;
+    std::vector<double> sc(s->M);
+    qpe::hip_check(hipMemcpyAsync(sc.data(), s->scale, s->M * sizeof(double), hipMemcpyDeviceToHost, s->stream), "scale copy");
+    qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+    for (uint64_t j = 0; j < s->M; ++j)
+        if (sc[j] != 1.0) qpe::scale_block(s->dtype, s->amp, j, s->D, sc[j], s->stream);
+    set_ones(s->scale, s->M, s->stream);
+    s->scale_pending = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qpe_last_error(void) { return g_last_error.c_str(); }
+
+int qpe_hamiltonian_create(qpe_hamiltonian_t *out) {
+    return guarded([&] {
+        require(out != nullptr, QPE_ERR_ARG, "null output pointer");
+        *out = new qpe_hamiltonian();
+    });
+}
+
+int qpe_hamiltonian_destroy(qpe_hamiltonian_t h) {
+    return guarded([&] { delete h; });
+}
+
+int qpe_hamiltonian_add_one_body_term(qpe_hamiltonian_t h, int p, int q, double coeff) {
+    return guarded([&] {
+        check_ham(h);
+        require(p >= 1 && q >= 1 && p >= q, QPE_ERR_ARG, "one-body indices must be 1-based with p >= q");
+        require(std::isfinite(coeff), QPE_ERR_ARG, "non-finite coefficient");
+        h->h.AddOneBodyTerm(p, q, coeff);
+        ++h->version;
+    });
+}
+
+int qpe_hamiltonian_add_two_body_term(qpe_hamiltonian_t h, int p, int q, int r, int s, double coeff) {
+    return guarded([&] {
+        check_ham(h);
+        require(p >= 1 && q >= 1 && r >= 1 && s >= 1, QPE_ERR_ARG, "two-body indices must be 1-based");
+        require(std::isfinite(coeff), QPE_ERR_ARG, "non-finite coefficient");
+        h->h.AddTwoBodyTerm(p, q, r, s, coeff);
+        ++h->version;
+    });
+}
+
+int qpe_hamiltonian_set_num_orbitals(qpe_hamiltonian_t h, unsigned n) {
+    return guarded([&] {
+        check_ham(h);
+        require(n >= 1 && n <= 32, QPE_ERR_ARG, "num_orbitals must be in [1, 32]");
+        h->h.SetNumOrbitals(n);
+        ++h->version;
+    });
+}
+
+int qpe_hamiltonian_set_num_occupied(qpe_hamiltonian_t h, unsigned e) {
+    return guarded([&] {
+        check_ham(h);
+        h->h.SetNumOccupied(e);
+        ++h->version;
+    });
+}
+
+int qpe_hamiltonian_index_terms(qpe_hamiltonian_t h) {
+    return guarded([&] {
+        check_ham(h);
+        require(h->h.num_orbitals() >= 1, QPE_ERR_STATE, "set num_orbitals before IndexTerms");
+        h->h.IndexTerms();
+        ++h->version;
+    });
+}
+
+int qpe_hamiltonian_set_timestep(qpe_hamiltonian_t h, double t) {
+    return guarded([&] {
+        check_ham(h);
+        require(std::isfinite(t), QPE_ERR_ARG, "non-finite timestep");
+        h->h.SetTimestep(t);
+        ++h->version;
+    });
+}
+
+int qpe_hamiltonian_term_counts(qpe_hamiltonian_t h, uint64_t counts[5]) {
+    return guarded([&] {
+        check_ham(h);
+        require(counts != nullptr, QPE_ERR_ARG, "null counts");
+        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+        for (int c = 0; c < 5; ++c) counts[c] = h->h.segmented()[c].size();
+    });
+}
+
+int qpe_hamiltonian_get_terms(qpe_hamiltonian_t h, int cls, int32_t *idx, double *coeff, uint64_t cap) {
+    return guarded([&] {
+        check_ham(h);
+        require(cls >= 0 && cls < 5, QPE_ERR_ARG, "term class out of range");
+        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+        const auto &v = h->h.segmented()[cls];
+        require(cap >= v.size(), QPE_ERR_ARG, "capacity too small");
+        for (size_t i = 0; i < v.size(); ++i) {
+            for (int k = 0; k < 4; ++k)
+                if (idx) idx[4 * i + k] = v[i].idx[k];
+            if (coeff) coeff[i] = v[i].coeff;
+        }
+    });
+}
+
+int qpe_state_create(qpe_state_t *out, unsigned precision, unsigned num_orbitals, unsigned num_occupied, int dtype) {
+    return guarded([&] {
+        require(out != nullptr, QPE_ERR_ARG, "null output pointer");
+        *out = nullptr;
+        require(precision >= 1 && precision <= 20, QPE_ERR_ARG, "precision must be in [1, 20]");
+        require(num_orbitals >= 2 && num_orbitals <= 32, QPE_ERR_ARG, "num_orbitals must be in [2, 32]");
+        require(num_occupied >= 1 && num_occupied < num_orbitals, QPE_ERR_ARG, "need 1 <= num_occupied < num_orbitals");
+        require(dtype == QPE_COMPLEX128 || dtype == QPE_COMPLEX64, QPE_ERR_ARG, "unknown dtype");
+        uint64_t C = choose((int)num_orbitals, (int)num_occupied);
+        require(C >= 1 && C <= 65535, QPE_ERR_ARG, "C(num_orbitals, num_occupied) exceeds 65535");
+        std::unique_ptr<qpe_state> s(new qpe_state());
+        s->m = precision; s->n = num_orbitals; s->e = num_occupied; s->dtype = dtype;
+        s->C = (uint32_t)C; s->D = C * C; s->M = 1ull << precision;
+        s->elem = dtype == QPE_COMPLEX128 ? 16 : 8;
+        s->renorm = dtype == QPE_COMPLEX64 ? 1 : 0;
+        const uint64_t bytes = s->M * s->D * s->elem;
+        int ndev = 0;
+        require(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, QPE_ERR_HIP, "no HIP device");
+        size_t free_b = 0, total_b = 0;
+        qpe::hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        require(bytes + (64ull << 20) < free_b, QPE_ERR_OOM, "state does not fit in free device memory");
+        qpe::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+        qpe::hip_check(hipMalloc(&s->amp, bytes), "hipMalloc(state)");
+        qpe::hip_check(hipMemsetAsync(s->amp, 0, bytes, s->stream), "zero state");
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->norms), s->M * sizeof(double)), "hipMalloc(norms)");
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->scale), s->M * sizeof(double)), "hipMalloc(scale)");
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->probs), s->M * sizeof(double)), "hipMalloc(probs)");
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->pk_partial), (size_t)kPkGridMax * s->M * sizeof(double)),
+                       "hipMalloc(pk)");
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->norm_partials), kNormParts * sizeof(double)), "hipMalloc");
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->status), sizeof(int)), "hipMalloc(status)");
+        qpe::hip_check(hipMemsetAsync(s->status, 0, sizeof(int), s->stream), "zero status");
+        qpe::hip_check(hipMemsetAsync(s->norms, 0, s->M * sizeof(double), s->stream), "zero norms");
+        // forward DFT twiddles e^{-2 pi i k / M}
+        const uint64_t half = std::max<uint64_t>(1, s->M / 2);
+        std::vector<double> tw(2 * half);
+        for (uint64_t k = 0; k < half; ++k) {
+            long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)s->M;
+            tw[2 * k] = (double)std::cos(a);
+            tw[2 * k + 1] = (double)-std::sin(a);
+        }
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->twiddles), tw.size() * sizeof(double)), "hipMalloc(tw)");
+        qpe::hip_check(hipMemcpyAsync(s->twiddles, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
+                       "twiddles");
+        set_ones(s->scale, s->M, s->stream);
+        s->max_lds = qpe::device_max_lds();
+        *out = s.release();
+    });
+}
+
+int qpe_state_destroy(qpe_state_t s) {
+    return guarded([&] {
+        if (s) (void)hipStreamSynchronize(s->stream);
+        delete s;
+    });
+}
+
+int qpe_state_size(qpe_state_t s, uint64_t *n) {
+    return guarded([&] {
+        check_state(s);
+        require(n != nullptr, QPE_ERR_ARG, "null output");
+        *n = s->M * s->D;
+    });
+}
+
+int qpe_state_set(qpe_state_t s, uint64_t offset, uint64_t count, const double *amps) {
+    return guarded([&] {
+        check_state(s);
+        require(amps != nullptr || count == 0, QPE_ERR_ARG, "null amplitudes");
+        require(offset <= s->M * s->D && count <= s->M * s->D - offset, QPE_ERR_ARG, "range outside the state");
+        if (count == 0) return;
+        materialize_scale(s);
+        const uint64_t chunk = 1ull << 26;
+        double *staging = nullptr;
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&staging), std::min(count, chunk) * 2 * sizeof(double)), "staging");
+        try {
+            for (uint64_t done = 0; done < count; done += chunk) {
+                uint64_t c = std::min(chunk, count - done);
+                qpe::hip_check(hipMemcpyAsync(staging, amps + 2 * done, c * 2 * sizeof(double), hipMemcpyHostToDevice,
+                                              s->stream), "upload");
+                qpe::scatter_in(s->dtype, s->amp, staging, offset + done, c, s->C, s->stream);
+            }
+            qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+        } catch (...) {
+            (void)hipFree(staging);
+            throw;
+        }
+        (void)hipFree(staging);
+        s->pk_valid = false;
+    });
+}
+
+int qpe_state_get(qpe_state_t s, uint64_t offset, uint64_t count, double *amps) {
+    return guarded([&] {
+        check_state(s);
+        require(amps != nullptr || count == 0, QPE_ERR_ARG, "null amplitudes");
+        require(offset <= s->M * s->D && count <= s->M * s->D - offset, QPE_ERR_ARG, "range outside the state");
+        if (count == 0) return;
+        materialize_scale(s);
+        const uint64_t chunk = 1ull << 26;
+        double *staging = nullptr;
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&staging), std::min(count, chunk) * 2 * sizeof(double)), "staging");
+        try {
+            for (uint64_t done = 0; done < count; done += chunk) {
+                uint64_t c = std::min(chunk, count - done);
+                qpe::gather_out(s->dtype, s->amp, staging, offset + done, c, s->C, s->stream);
+                qpe::hip_check(hipMemcpyAsync(amps + 2 * done, staging, c * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                              s->stream), "download");
+                qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+            }
+        } catch (...) {
+            (void)hipFree(staging);
+            throw;
+        }
+        (void)hipFree(staging);
+    });
+}
+
+int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor) {
+    return guarded([&] {
+        check_state(s);
+        require(block < s->M, QPE_ERR_ARG, "block out of range");
+        require(std::isfinite(factor), QPE_ERR_ARG, "non-finite factor");
+        materialize_scale(s);
+        qpe::scale_block(s->dtype, s->amp, block, s->D, factor, s->stream);
+        qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+        s->pk_valid = false;
+    });
+}
+
+int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
+    return guarded([&] {
+        check_state(s);
+        switch (option) {
+        case QPE_OPT_RENORMALIZE: s->renorm = value ? 1 : 0; break;
+        case QPE_OPT_ENGINE:
+            require(value == 0, QPE_ERR_ARG, "only the tile engine (0) is available");
+            s->engine = (int)value;
+            break;
+        case QPE_OPT_MAX_TILE_BYTES:
+            require(value >= 0, QPE_ERR_ARG, "negative tile budget");
+            s->max_tile_bytes = value;
+            s->plan_version = ~0ull;
+            break;
+        default: throw ApiError(QPE_ERR_ARG, "unknown option");
+        }
+    });
+}
+
+int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h) {
+    return guarded([&] {
+        check_state(s);
+        check_ham(h);
+        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+        require(h->h.num_orbitals() == s->n && h->h.num_occupied() == s->e, QPE_ERR_ARG,
+                "hamiltonian (num_orbitals, num_occupied) does not match the state");
+        materialize_scale(s);
+        if (!s->plan || s->plan_h != h || s->plan_version != h->version) {
+            size_t budget = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
+            if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
+            qpe::Plan p;
+            try {
+                p = qpe::build_plan(h->h, s->elem, budget);
+            } catch (const std::runtime_error &e) {
+                throw ApiError(QPE_ERR_ARG, e.what());
+            }
+            std::unique_ptr<qpe::DevicePlan> dp(new qpe::DevicePlan());
+            dp->upload(p, s->dtype == QPE_COMPLEX128 ? 0 : 1, s->stream);
+            s->plan = std::move(dp);
+            s->plan_h = h;
+            s->plan_version = h->version;
+        }
+        qpe::hip_check(hipMemsetAsync(s->status, 0, sizeof(int), s->stream), "zero status");
+        set_ones(s->scale, s->M, s->stream);
+        qpe::block_norm(s->dtype == QPE_COMPLEX128 ? 0 : 1, s->D, s->amp, 0, s->norms, s->scale, s->norm_partials, kNormParts, s->stream);
+        hipEvent_t e0, e1;
+        qpe::hip_check(hipEventCreate(&e0), "event");
+        qpe::hip_check(hipEventCreate(&e1), "event");
+        qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
+        qpe::evolve(*s->plan, s->amp, s->M, s->norms, s->scale, s->renorm, s->status, s->stream, &s->launches);
+        qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
+        qpe::hip_check(hipEventSynchronize(e1), "evolve sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->t_evolve = ms;
+        s->passes = (int)s->plan->plan.passes.size();
+        s->scale_pending = s->renorm != 0;
+        s->pk_valid = false;
+        int status = 0;
+        qpe::hip_check(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost), "status copy");
+        require(status == 0, QPE_ERR_NUMERIC, "non-finite amplitudes during evolution");
+    });
+}
+
+int qpe_qift(qpe_state_t s) {
+    return guarded([&] {
+        check_state(s);
+        hipEvent_t e0, e1;
+        qpe::hip_check(hipEventCreate(&e0), "event");
+        qpe::hip_check(hipEventCreate(&e1), "event");
+        qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
+        qpe::qift(s->dtype, s->amp, (int)s->m, s->D, s->scale, s->pk_partial, &s->pk_grid, s->twiddles,
+                  (size_t)s->max_lds, s->stream);
+        qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
+        qpe::hip_check(hipEventSynchronize(e1), "qift sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->t_qift = ms;
+        set_ones(s->scale, s->M, s->stream);  // the QIFT applied them
+        s->scale_pending = false;
+        s->pk_valid = true;
+    });
+}
+
+int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs) {
+    return guarded([&] {
+        check_state(s);
+        require(probs != nullptr, QPE_ERR_ARG, "null probs");
+        hipEvent_t e0, e1;
+        qpe::hip_check(hipEventCreate(&e0), "event");
+        qpe::hip_check(hipEventCreate(&e1), "event");
+        qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
+        if (s->pk_valid) {
+            qpe::reduce_pk(s->pk_partial, s->pk_grid, (int)s->m, s->probs, s->stream);
+        } else {
+            // P(k) = squared norm of block k of the current register
+            materialize_scale(s);
+            for (uint64_t j = 0; j < s->M; ++j)
+                qpe::block_norm(s->dtype == QPE_COMPLEX128 ? 0 : 1, s->D, s->amp, j, s->probs, s->scale, s->norm_partials, kNormParts, s->stream);
+            set_ones(s->scale, s->M, s->stream);
+        }
+        qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
+        qpe::hip_check(hipMemcpyAsync(probs, s->probs, s->M * sizeof(double), hipMemcpyDeviceToHost, s->stream), "probs");
+        qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->t_pk = ms;
+    });
+}
+
+int qpe_state_timings(qpe_state_t s, double out[5]) {
+    return guarded([&] {
+        check_state(s);
+        require(out != nullptr, QPE_ERR_ARG, "null output");
+        out[0] = s->t_evolve;
+        out[1] = s->t_qift;
+        out[2] = s->t_pk;
+        out[3] = s->passes;
+        out[4] = s->launches;
+    });
+}
+
+int qpe_state_block_norms(qpe_state_t s, double *norms) {
+    return guarded([&] {
+        check_state(s);
+        require(norms != nullptr, QPE_ERR_ARG, "null output");
+        qpe::hip_check(hipMemcpyAsync(norms, s->norms, s->M * sizeof(double), hipMemcpyDeviceToHost, s->stream), "norms");
+        qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int qpe_debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *idx01, uint32_t *idx10, uint32_t *parity,
+                    uint64_t cap, uint64_t *count) {
+    return guarded([&] {
+        require(args && idx01 && idx10 && parity && count, QPE_ERR_ARG, "null argument");
+        require(n >= 2 && n <= 32 && e >= 1 && e < n, QPE_ERR_ARG, "bad (n, e)");
+        uint64_t c;
+        if (kind == 2) c = choose(n - 2, e - 1);
+        else if (kind == 3) c = e >= 2 ? choose(n - 3, e - 2) : 0;
+        else if (kind == 4) c = e >= 2 ? choose(n - 4, e - 2) : 0;
+        else throw ApiError(QPE_ERR_ARG, "kind must be 2 (CA), 3 (CNA) or 4 (CCAA)");
+        int na = kind == 2 ? 2 : kind == 3 ? 3 : 4;
+        for (int i = 0; i < na; ++i) require(args[i] >= 0 && args[i] < n, QPE_ERR_ARG, "orbital out of range");
+        *count = c;
+        require(cap >= c, QPE_ERR_ARG, "capacity too small");
+        if (c == 0) return;
+        int32_t a4[4] = {args[0], args[1], na > 2 ? args[2] : 0, na > 3 ? args[3] : 0};
+        uint32_t *d = nullptr;
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&d), 3 * c * sizeof(uint32_t)), "hipMalloc");
+        try {
+            qpe::debug_pairs(kind, n, e, a4, d, d + c, d + 2 * c, c, nullptr);
+            qpe::hip_check(hipMemcpy(idx01, d, c * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy");
+            qpe::hip_check(hipMemcpy(idx10, d + c, c * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy");
+            qpe::hip_check(hipMemcpy(parity, d + 2 * c, c * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy");
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipFree(d);
+    });
+}
+
+}  // extern "C"

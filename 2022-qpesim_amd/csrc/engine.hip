@@ -1,0 +1,671 @@
+// gfx950 kernels of the QPE hot path and their launchers.
+//
+//   pass_kernel   one tile pass of a Trotter step (COL or ROW tiles in LDS)
+//   norm_finalize FP64 compensated block norm + renormalisation factor
+//   qift_kernel   batched radix-2 DFT along the ancilla axis in LDS, with the
+//                 1/sqrt(M) normalisation and the P(k) partial sums fused
+//
+// Amplitudes: complex128 (double2) or complex64 (float2).  Angles, their
+// cos/sin, norms and P(k) are always FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "combinadic.hpp"
+#include "engine.hpp"
+
+namespace qpe {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+constexpr int kThreads = 512;
+constexpr int kMaxLines = 64;
+constexpr int kBinomWords = kBinomRows * kBinomCols;
+
+template <typename R> struct Vec;
+template <> struct Vec<double> { using type = double2; };
+template <> struct Vec<float> { using type = float2; };
+
+// Neumaier compensated accumulation
+struct KSum {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double x) {
+        double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+// 2x2 symmetric rotation {c, i*mu; i*mu, c} on the pair (u, v)
+template <typename V, typename R>
+__device__ inline void rotate(V &u, V &v, R c, R mu) {
+    V nu, nv;
+    nu.x = c * u.x - mu * v.y;
+    nu.y = c * u.y + mu * v.x;
+    nv.x = c * v.x - mu * u.y;
+    nv.y = c * v.y + mu * u.x;
+    u = nu;
+    v = nv;
+}
+
+struct PassArgs {
+    const uint32_t *binom;
+    const uint32_t *strings;
+    const double *theta_a, *theta_b, *cross;
+    const GroupDev *groups;
+    const double *grp_a, *grp_b;
+    const TermDev *terms;
+    const OpDev *ops;
+    const ItemDev *items;
+    const uint32_t *col_list;
+    double *partials;
+    const double *scale;
+    int n, e;
+    uint32_t C, ca_pairs, ccaa_pairs;
+    int renorm;
+};
+
+// Number of lanes that share one determinant pair (power of two, <= lines).
+__device__ inline int lanes_per_pair(int npairs, int nlines) {
+    int lp = 1;
+    while (lp * 2 <= nlines && npairs * lp * 2 <= kThreads) lp *= 2;
+    return lp;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
+    using V = typename Vec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t *tab = reinterpret_cast<uint32_t *>(smem);                 // binomials
+    uint32_t *line_rank = tab + kBinomWords;                            // [kMaxLines]
+    uint32_t *line_str = line_rank + kMaxLines;                         // [kMaxLines]
+    int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines] slot01 | slot10<<8 | par<<16
+    int *pair_count = pair_list + kMaxLines;                            // [1] (+pad)
+    double *red = reinterpret_cast<double *>(pair_count + 4);           // [kThreads/64 * 2]
+    V *tile = reinterpret_cast<V *>(smem + 4096);
+
+    const int tid = threadIdx.x;
+    const ItemDev it = A.items[pass.item_begin + blockIdx.x];
+    const int nl = (int)it.count;
+    const uint32_t C = A.C;
+    const bool col = pass.kind == PASS_COL;
+
+    for (int i = tid; i < kBinomWords; i += kThreads) tab[i] = A.binom[i];
+    if (tid < nl) {
+        uint32_t r = col ? A.col_list[it.begin + tid] : it.begin + tid;
+        line_rank[tid] = r;
+        line_str[tid] = A.strings[r];
+    }
+    __syncthreads();
+
+    // ---- load the tile ---------------------------------------------------
+    const bool from_prev = pass.flags & PF_FROM_PREV;
+    const V *src = from_prev ? prev : cur;
+    R sc = (R)1;
+    if (from_prev && A.renorm) sc = (R)A.scale[j - 1];
+    const uint32_t total = (uint32_t)nl * C;
+    if (col) {
+        for (uint32_t idx = tid; idx < total; idx += kThreads) {
+            uint32_t l = idx / C, a = idx - l * C;
+            V v = src[(uint64_t)line_rank[l] * C + a];
+            v.x *= sc; v.y *= sc;
+            tile[idx] = v;
+        }
+    } else {
+        const uint32_t r0 = it.begin;
+        for (uint32_t idx = tid; idx < total; idx += kThreads) {
+            uint32_t b = idx / nl, l = idx - b * nl;
+            V v = src[(uint64_t)b * C + r0 + l];
+            v.x *= sc; v.y *= sc;
+            tile[l * C + b] = v;
+        }
+    }
+
+    // ---- apply the ops in order -------------------------------------------
+    for (int o = pass.op_begin; o < pass.op_end; ++o) {
+        const OpDev op = A.ops[o];
+        __syncthreads();
+        if (op.type == OP_DIAG) {
+            for (uint32_t idx = tid; idx < total; idx += kThreads) {
+                uint32_t l = idx / C, a = idx - l * C;
+                uint32_t b = line_rank[l];
+                uint32_t sa = A.strings[a];
+                double th = A.theta_a[a] + A.theta_b[b];
+                while (sa) {
+                    int p = __builtin_ctz(sa);
+                    th += A.cross[(uint64_t)p * C + b];
+                    sa &= sa - 1;
+                }
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                V v = tile[idx], w;
+                R c = (R)cs, s = (R)(-sn);
+                w.x = c * v.x - s * v.y;
+                w.y = c * v.y + s * v.x;
+                tile[idx] = w;
+            }
+        } else if (op.type == OP_GA || op.type == OP_GB) {
+            const GroupDev g = A.groups[op.index];
+            const double *ga = A.grp_a + 2ull * op.index * A.ca_pairs;
+            const double *gb = A.grp_b + 2ull * op.index * C;
+            const int np = (int)A.ca_pairs;
+            const int lp = lanes_per_pair(np, nl);
+            for (int k = tid / lp; k < np; k += kThreads / lp) {
+                uint32_t rx, ry, par;
+                ca_pair((uint32_t)k, g.x, g.y, A.n, A.e, tab, &rx, &ry, &par);
+                double ca = ga[2 * k], sa = ga[2 * k + 1];
+                for (int l = tid % lp; l < nl; l += lp) {
+                    uint32_t other = line_rank[l];
+                    double cb = gb[2 * other], sb = gb[2 * other + 1];
+                    double c = ca * cb - sa * sb;
+                    double s = sa * cb + ca * sb;
+                    double mu = par ? s : -s;
+                    rotate(tile[l * C + rx], tile[l * C + ry], (R)c, (R)mu);
+                }
+            }
+        } else if (op.type == OP_AAAA || op.type == OP_BBBB) {
+            const TermDev T = A.terms[op.index];
+            const int np = (int)A.ccaa_pairs;
+            const int lp = lanes_per_pair(np, nl);
+            const R c = (R)T.cs;
+            for (int k = tid / lp; k < np; k += kThreads / lp) {
+                uint32_t r01, r10, par;
+                ccaa_pair((uint32_t)k, T.p, T.q, T.r, T.s, A.n, A.e, tab, &r01, &r10, &par);
+                const R mu = (R)(par ? T.sn : -T.sn);
+                for (int l = tid % lp; l < nl; l += lp) rotate(tile[l * C + r01], tile[l * C + r10], c, mu);
+            }
+        } else {  // OP_ABBA on a COL tile: alpha CA(p,s) x beta column pairs (q,r)
+            const TermDev T = A.terms[op.index];
+            if (tid == 0) *pair_count = 0;
+            __syncthreads();
+            if (tid < nl) {
+                uint32_t s = line_str[tid];
+                if (((s >> T.r) & 1u) && !((s >> T.q) & 1u)) {
+                    uint32_t ps = s ^ (1u << T.q) ^ (1u << T.r);
+                    int partner = -1;
+                    for (int l = 0; l < nl; ++l)
+                        if (line_str[l] == ps) partner = l;
+                    if (partner >= 0) {
+                        int parb = popcnt(s & between_mask(T.q, T.r)) & 1;
+                        int slot = atomicAdd(pair_count, 1);
+                        pair_list[slot] = tid | (partner << 8) | (parb << 16);
+                    }
+                }
+            }
+            __syncthreads();
+            const int ncp = *pair_count;
+            const int np = (int)A.ca_pairs;
+            const int lp = lanes_per_pair(np, ncp);
+            const R c = (R)T.cs;
+            for (int k = tid / lp; k < np; k += kThreads / lp) {
+                uint32_t rs_, rp_, para;
+                ca_pair((uint32_t)k, T.s, T.p, A.n, A.e, tab, &rs_, &rp_, &para);
+                for (int x = tid % lp; x < ncp; x += lp) {
+                    int pl = pair_list[x];
+                    int l01 = pl & 0xff, l10 = (pl >> 8) & 0xff, parb = (pl >> 16) & 1;
+                    const R mu = (R)(((para + parb) & 1) ? T.sn : -T.sn);
+                    rotate(tile[l01 * C + rs_], tile[l10 * C + rp_], c, mu);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- FP64 norm of the tile (last pass of a step) ----------------------
+    if (pass.flags & PF_EMIT_NORM) {
+        KSum acc;
+        for (uint32_t idx = tid; idx < total; idx += kThreads) {
+            V v = tile[idx];
+            acc.add((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        }
+        double s = acc.s, c = acc.c;
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_down(s, off);
+            c += __shfl_down(c, off);
+        }
+        if ((tid & 63) == 0) {
+            red[2 * (tid >> 6)] = s;
+            red[2 * (tid >> 6) + 1] = c;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            KSum tot;
+            for (int w = 0; w < kThreads / 64; ++w) {
+                tot.add(red[2 * w]);
+                tot.add(red[2 * w + 1]);
+            }
+            A.partials[blockIdx.x] = tot.value();
+        }
+    }
+
+    // ---- store the tile -----------------------------------------------------
+    if (col) {
+        for (uint32_t idx = tid; idx < total; idx += kThreads) {
+            uint32_t l = idx / C, a = idx - l * C;
+            cur[(uint64_t)line_rank[l] * C + a] = tile[idx];
+        }
+    } else {
+        const uint32_t r0 = it.begin;
+        for (uint32_t idx = tid; idx < total; idx += kThreads) {
+            uint32_t b = idx / nl, l = idx - b * nl;
+            cur[(uint64_t)b * C + r0 + l] = tile[l * C + b];
+        }
+    }
+}
+
+// sum the per-workgroup partials of block j; set norms[j] and scale[j]
+__global__ void norm_finalize(const double *partials, int count, uint64_t j, double *norms, double *scale,
+                              int *status) {
+    __shared__ double red[2 * 16];
+    KSum acc;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) acc.add(partials[i]);
+    double s = acc.s, c = acc.c;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off);
+        c += __shfl_down(c, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = s;
+        red[2 * (threadIdx.x >> 6) + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KSum tot;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) {
+            tot.add(red[2 * w]);
+            tot.add(red[2 * w + 1]);
+        }
+        double v = tot.value();
+        norms[j] = v;
+        double f = 1.0;
+        if (j > 0 && v > 0.0 && isfinite(v)) f = sqrt(norms[0] / v);
+        scale[j] = f;
+        if (!isfinite(v) && status) *status = 1;
+    }
+}
+
+// squared norm of one block, partial sums per workgroup
+template <typename R>
+__global__ void __launch_bounds__(256) block_norm_kernel(const typename Vec<R>::type *blk, uint64_t D, double *partials) {
+    __shared__ double red[2 * 4];
+    KSum acc;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < D; i += (uint64_t)gridDim.x * 256ull) {
+        auto v = blk[i];
+        acc.add((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+    }
+    double s = acc.s, c = acc.c;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off);
+        c += __shfl_down(c, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = s;
+        red[2 * (threadIdx.x >> 6) + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KSum tot;
+        for (int w = 0; w < 4; ++w) {
+            tot.add(red[2 * w]);
+            tot.add(red[2 * w + 1]);
+        }
+        partials[blockIdx.x] = tot.value();
+    }
+}
+
+// ---- QIFT ------------------------------------------------------------------
+constexpr int kQThreads = 512;
+constexpr int kQAcc = 32;   // P(k) accumulators per thread: M*w <= kQThreads*kQAcc
+
+__device__ inline uint32_t bitrev(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
+
+template <typename R>
+__global__ void __launch_bounds__(kQThreads)
+qift_kernel(typename Vec<R>::type *state, uint64_t D, int m, int w, const double *scale, const double2 *twiddles,
+            double *pk_partial) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t M = 1u << m;
+    double2 *tw = reinterpret_cast<double2 *>(smem);          // [M/2]
+    double2 *tile = tw + (M / 2 > 0 ? M / 2 : 1);              // [M][w]
+    const int tid = threadIdx.x;
+    const uint32_t elems = M * (uint32_t)w;
+    const int nacc = (int)((elems + kQThreads - 1) / kQThreads);
+    const double inv = 1.0 / sqrt((double)M);
+
+    for (uint32_t i = tid; i < M / 2; i += kQThreads) tw[i] = twiddles[i];
+    double acc[kQAcc], cmp[kQAcc];
+#pragma unroll
+    for (int i = 0; i < kQAcc; ++i) { acc[i] = 0.0; cmp[i] = 0.0; }
+
+    const uint64_t ntiles = (D + w - 1) / w;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t d0 = t * (uint64_t)w;
+        const int cols = (int)((D - d0) < (uint64_t)w ? (D - d0) : (uint64_t)w);
+        __syncthreads();
+        for (uint32_t idx = tid; idx < elems; idx += kQThreads) {
+            uint32_t jj = idx / w, c = idx - jj * w;
+            double2 v = make_double2(0.0, 0.0);
+            if ((int)c < cols) {
+                auto x = state[(uint64_t)jj * D + d0 + c];
+                double f = scale[jj];
+                v.x = (double)x.x * f;
+                v.y = (double)x.y * f;
+            }
+            tile[bitrev(jj, m) * w + c] = v;
+        }
+        for (uint32_t len = 2; len <= M; len <<= 1) {
+            __syncthreads();
+            const uint32_t half = len >> 1, stride = M / len;
+            for (uint32_t idx = tid; idx < (M / 2) * w; idx += kQThreads) {
+                uint32_t c = idx % w, b = idx / w;
+                uint32_t grp = b / half, kk = b - grp * half;
+                uint32_t i0 = grp * len + kk, i1 = i0 + half;
+                double2 u = tile[i0 * w + c], v = tile[i1 * w + c], z = tw[kk * stride];
+                double2 vz = make_double2(v.x * z.x - v.y * z.y, v.x * z.y + v.y * z.x);
+                tile[i0 * w + c] = make_double2(u.x + vz.x, u.y + vz.y);
+                tile[i1 * w + c] = make_double2(u.x - vz.x, u.y - vz.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kQAcc; ++i) {
+            uint32_t idx = tid + (uint32_t)i * kQThreads;
+            if (i < nacc && idx < elems) {
+                uint32_t kk = idx / w, c = idx - kk * w;
+                if ((int)c < cols) {
+                    double2 y = tile[idx];
+                    y.x *= inv;
+                    y.y *= inv;
+                    auto &o = state[(uint64_t)kk * D + d0 + c];
+                    o.x = (R)y.x;
+                    o.y = (R)y.y;
+                    double p = y.x * y.x + y.y * y.y;
+                    double s = acc[i] + p;
+                    if (fabs(acc[i]) >= p) cmp[i] += (acc[i] - s) + p;
+                    else cmp[i] += (p - s) + acc[i];
+                    acc[i] = s;
+                }
+            }
+        }
+    }
+    // per-workgroup P(k): combine the w lanes of each k in a fixed order
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(tile);
+#pragma unroll
+    for (int i = 0; i < kQAcc; ++i) {
+        uint32_t idx = tid + (uint32_t)i * kQThreads;
+        if (i < nacc && idx < elems) red[idx] = acc[i] + cmp[i];
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < M; k += kQThreads) {
+        double s = 0.0, c = 0.0;
+        for (int q = 0; q < w; ++q) {
+            double x = red[k * w + q];
+            double t2 = s + x;
+            if (fabs(s) >= fabs(x)) c += (s - t2) + x;
+            else c += (x - t2) + s;
+            s = t2;
+        }
+        pk_partial[(uint64_t)blockIdx.x * M + k] = s + c;
+    }
+}
+
+__global__ void pk_reduce(const double *pk_partial, int grid, uint32_t M, double *probs) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    double s = 0.0, c = 0.0;
+    for (int g = 0; g < grid; ++g) {
+        double x = pk_partial[(uint64_t)g * M + k];
+        double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    probs[k] = s + c;
+}
+
+// ---- host <-> device layout ---------------------------------------------
+template <typename R>
+__global__ void scatter_kernel(typename Vec<R>::type *state, const double2 *staging, uint64_t offset, uint64_t count,
+                               uint32_t C) {
+    const uint64_t D = (uint64_t)C * C;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256ull) {
+        uint64_t ref = offset + i, jj = ref / D, rem = ref - jj * D;
+        uint64_t a = rem / C, b = rem - a * C;
+        double2 v = staging[i];
+        auto &o = state[jj * D + b * C + a];
+        o.x = (R)v.x;
+        o.y = (R)v.y;
+    }
+}
+
+template <typename R>
+__global__ void gather_kernel(const typename Vec<R>::type *state, double2 *staging, uint64_t offset, uint64_t count,
+                              uint32_t C) {
+    const uint64_t D = (uint64_t)C * C;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256ull) {
+        uint64_t ref = offset + i, jj = ref / D, rem = ref - jj * D;
+        uint64_t a = rem / C, b = rem - a * C;
+        auto v = state[jj * D + b * C + a];
+        staging[i] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
+template <typename R>
+__global__ void scale_kernel(typename Vec<R>::type *blk, uint64_t D, double f) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < D; i += (uint64_t)gridDim.x * 256ull) {
+        auto v = blk[i];
+        v.x = (R)((double)v.x * f);
+        v.y = (R)((double)v.y * f);
+        blk[i] = v;
+    }
+}
+
+__global__ void debug_pairs_kernel(int kind, int n, int e, int a0, int a1, int a2, int a3, uint32_t *o1,
+                                   uint32_t *o2, uint32_t *op, uint64_t count) {
+    __shared__ uint32_t tab[kBinomWords];
+    if (threadIdx.x == 0) fill_binomials(tab);
+    __syncthreads();
+    uint64_t k = blockIdx.x * 256ull + threadIdx.x;
+    if (k >= count) return;
+    uint32_t r1 = 0, r2 = 0, par = 0;
+    if (kind == 2) ca_pair((uint32_t)k, a1, a0, n, e, tab, &r1, &r2, &par);
+    else if (kind == 3) cna_pair((uint32_t)k, a0, a1, a2, n, e, tab, &r1, &r2, &par);
+    else ccaa_pair((uint32_t)k, a0, a1, a2, a3, n, e, tab, &r1, &r2, &par);
+    o1[k] = r1;
+    o2[k] = r2;
+    op[k] = par;
+}
+
+unsigned grid_for(uint64_t n) {
+    uint64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    return (unsigned)(g ? g : 1);
+}
+
+template <typename T>
+void to_device(T **dst, const std::vector<T> &v, hipStream_t s) {
+    size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    hip_check(hipMalloc(reinterpret_cast<void **>(dst), bytes), "hipMalloc(plan)");
+    if (!v.empty()) hip_check(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "plan upload");
+}
+
+}  // namespace
+
+int device_max_lds() {
+    int dev = 0, v = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "max LDS");
+    return v;
+}
+
+void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
+    release();
+    plan = p;
+    dtype = dt;
+    std::vector<uint32_t> tab(kBinomWords);
+    fill_binomials(tab.data());
+    to_device(&binom, tab, s);
+    to_device(&strings, p.strings, s);
+    to_device(&theta_a, p.theta_a, s);
+    to_device(&theta_b, p.theta_b, s);
+    to_device(&cross, p.cross, s);
+    to_device(&groups, p.groups, s);
+    to_device(&grp_a, p.grp_a, s);
+    to_device(&grp_b, p.grp_b, s);
+    to_device(&terms, p.terms, s);
+    to_device(&ops, p.ops, s);
+    to_device(&items, p.items, s);
+    to_device(&col_list, p.col_list, s);
+    hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
+    int maxlines = 0;
+    for (const auto &ps : p.passes) maxlines = std::max(maxlines, ps.max_lines);
+    if (maxlines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
+    lds_bytes = 4096 + (size_t)maxlines * p.C * p.elem_bytes;
+    hip_check(hipStreamSynchronize(s), "plan upload sync");
+}
+
+void DevicePlan::release() {
+    void *ptrs[] = {binom, strings, theta_a, theta_b, cross, groups, grp_a, grp_b, terms, ops, items, col_list, partials};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    binom = nullptr; strings = nullptr; theta_a = theta_b = cross = nullptr; groups = nullptr;
+    grp_a = grp_b = nullptr; terms = nullptr; ops = nullptr; items = nullptr; col_list = nullptr; partials = nullptr;
+}
+
+template <typename R>
+static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
+                     int *status_flag, hipStream_t s, int *launches) {
+    using V = typename Vec<R>::type;
+    const Plan &p = dp.plan;
+    PassArgs A;
+    A.binom = dp.binom; A.strings = dp.strings;
+    A.theta_a = dp.theta_a; A.theta_b = dp.theta_b; A.cross = dp.cross;
+    A.groups = dp.groups; A.grp_a = dp.grp_a; A.grp_b = dp.grp_b;
+    A.terms = dp.terms; A.ops = dp.ops; A.items = dp.items; A.col_list = dp.col_list;
+    A.partials = dp.partials; A.scale = scale;
+    A.n = p.n; A.e = p.e; A.C = p.C; A.ca_pairs = p.ca_pairs; A.ccaa_pairs = p.ccaa_pairs;
+    A.renorm = renorm;
+    const uint64_t D = (uint64_t)p.C * p.C;
+    V *base = reinterpret_cast<V *>(state);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    int norm_items = 0;
+    for (const auto &ps : p.passes)
+        if (ps.flags & PF_EMIT_NORM) norm_items = ps.item_count;
+    for (uint64_t j = 1; j < M; ++j) {
+        V *prev = base + (j - 1) * D, *cur = base + j * D;
+        for (const auto &ps : p.passes) {
+            hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.lds_bytes, s, A, ps, prev, cur, j);
+        }
+        hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, j, norms,
+                           scale, status_flag);
+    }
+    hip_check(hipGetLastError(), "pass launch");
+    if (launches) *launches = (int)p.passes.size() + 1;
+}
+
+void evolve(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
+            int *status_flag, hipStream_t s, int *launches) {
+    if (dp.dtype == 0) evolve_t<double>(dp, state, M, norms, scale, renorm, status_flag, s, launches);
+    else evolve_t<float>(dp, state, M, norms, scale, renorm, status_flag, s, launches);
+}
+
+void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, double *scale, double *partials,
+                int nparts, hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(block_norm_kernel<double>, dim3(nparts), dim3(256), 0, s,
+                           reinterpret_cast<const double2 *>(state) + j * D, D, partials);
+    else
+        hipLaunchKernelGGL(block_norm_kernel<float>, dim3(nparts), dim3(256), 0, s,
+                           reinterpret_cast<const float2 *>(state) + j * D, D, partials);
+    hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)partials, nparts, j, norms, scale,
+                       (int *)nullptr);
+    hip_check(hipGetLastError(), "block norm launch");
+}
+
+void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial, int *grid_out,
+          double *twiddles, size_t max_lds, hipStream_t s) {
+    const uint32_t M = 1u << m;
+    const size_t tw_bytes = (size_t)std::max<uint32_t>(1, M / 2) * sizeof(double2);
+    int w = 1;
+    // widest column tile that fits LDS and the accumulator budget, capped at 32
+    while (w * 2 <= 32 && (size_t)M * (w * 2) <= (size_t)kQThreads * kQAcc &&
+           tw_bytes + (size_t)M * (w * 2) * sizeof(double2) <= max_lds)
+        w *= 2;
+    if (tw_bytes + (size_t)M * w * sizeof(double2) > max_lds || (size_t)M * w > (size_t)kQThreads * kQAcc)
+        throw std::runtime_error("2^m too large for the QIFT tile");
+    const uint64_t ntiles = (D + w - 1) / w;
+    int grid = (int)std::min<uint64_t>(ntiles, 1024);
+    size_t lds = tw_bytes + (size_t)M * w * sizeof(double2);
+    *grid_out = grid;
+    if (dtype == 0) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&qift_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(qift_kernel<double>, dim3(grid), dim3(kQThreads), lds, s, reinterpret_cast<double2 *>(state),
+                           D, m, w, scale, reinterpret_cast<const double2 *>(twiddles), pk_partial);
+    } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&qift_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(qift_kernel<float>, dim3(grid), dim3(kQThreads), lds, s, reinterpret_cast<float2 *>(state),
+                           D, m, w, scale, reinterpret_cast<const double2 *>(twiddles), pk_partial);
+    }
+    hip_check(hipGetLastError(), "qift launch");
+}
+
+void reduce_pk(const double *pk_partial, int grid, int m, double *probs_dev, hipStream_t s) {
+    const uint32_t M = 1u << m;
+    hipLaunchKernelGGL(pk_reduce, dim3((M + 255) / 256), dim3(256), 0, s, pk_partial, grid, M, probs_dev);
+    hip_check(hipGetLastError(), "pk reduce launch");
+}
+
+void scatter_in(int dtype, void *state, const double *staging, uint64_t offset, uint64_t count, uint32_t C,
+                hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(scatter_kernel<double>, dim3(grid_for(count)), dim3(256), 0, s,
+                           reinterpret_cast<double2 *>(state), reinterpret_cast<const double2 *>(staging), offset, count, C);
+    else
+        hipLaunchKernelGGL(scatter_kernel<float>, dim3(grid_for(count)), dim3(256), 0, s,
+                           reinterpret_cast<float2 *>(state), reinterpret_cast<const double2 *>(staging), offset, count, C);
+    hip_check(hipGetLastError(), "scatter launch");
+}
+
+void gather_out(int dtype, const void *state, double *staging, uint64_t offset, uint64_t count, uint32_t C,
+                hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(gather_kernel<double>, dim3(grid_for(count)), dim3(256), 0, s,
+                           reinterpret_cast<const double2 *>(state), reinterpret_cast<double2 *>(staging), offset, count, C);
+    else
+        hipLaunchKernelGGL(gather_kernel<float>, dim3(grid_for(count)), dim3(256), 0, s,
+                           reinterpret_cast<const float2 *>(state), reinterpret_cast<double2 *>(staging), offset, count, C);
+    hip_check(hipGetLastError(), "gather launch");
+}
+
+void scale_block(int dtype, void *state, uint64_t j, uint64_t D, double factor, hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for(D)), dim3(256), 0, s,
+                           reinterpret_cast<double2 *>(state) + j * D, D, factor);
+    else
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(D)), dim3(256), 0, s,
+                           reinterpret_cast<float2 *>(state) + j * D, D, factor);
+    hip_check(hipGetLastError(), "scale launch");
+}
+
+void debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *o1, uint32_t *o2, uint32_t *op,
+                 uint64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(debug_pairs_kernel, dim3(grid_for(count)), dim3(256), 0, s, kind, n, e, args[0], args[1],
+                       args[2], args[3], o1, o2, op, count);
+    hip_check(hipGetLastError(), "debug pairs launch");
+}
+
+}  // namespace qpe

@@ -1,0 +1,66 @@
+// Device engine: owns the device copy of a Plan and launches its passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "plan.hpp"
+
+namespace qpe {
+
+// throws std::runtime_error carrying the HIP error text
+void hip_check(hipError_t e, const char *what);
+
+struct DevicePlan {
+    int dtype = 0;           // 0 complex128, 1 complex64
+    Plan plan;
+    uint32_t *binom = nullptr;
+    uint32_t *strings = nullptr;
+    double *theta_a = nullptr, *theta_b = nullptr, *cross = nullptr;
+    GroupDev *groups = nullptr;
+    double *grp_a = nullptr, *grp_b = nullptr;
+    TermDev *terms = nullptr;
+    OpDev *ops = nullptr;
+    ItemDev *items = nullptr;
+    uint32_t *col_list = nullptr;
+    double *partials = nullptr;   // [max_items] norm partial sums
+    int block_threads = 512;
+    size_t lds_bytes = 0;         // dynamic LDS per pass launch
+
+    void upload(const Plan &p, int dtype, hipStream_t s);
+    void release();
+    ~DevicePlan() { release(); }
+};
+
+// One time_evolve_over_terms: blocks 1..M-1 of `state` (device, column-major
+// blocks of C*C amplitudes).  norms[M] and scale[M] are device arrays; norms[0]
+// and scale[0] must be set (block 0) before the call.  renorm selects whether
+// each block is rescaled to block 0's norm when it is read.
+void evolve(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale,
+            int renorm, int *status_flag, hipStream_t s, int *launches_per_step);
+
+// squared norm of block j into norms[j] (and scale[j] = 1)
+void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, double *scale,
+                double *partials, int nparts, hipStream_t s);
+
+// QIFT in place with the renormalisation factors applied, and P(k) partials.
+void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial,
+          int *grid_out, double *twiddles, size_t max_lds, hipStream_t s);
+void reduce_pk(const double *pk_partial, int grid, int m, double *probs_dev, hipStream_t s);
+
+// reference-order host data <-> device layout, with dtype conversion
+void scatter_in(int dtype, void *state, const double *staging, uint64_t offset, uint64_t count,
+                uint32_t C, hipStream_t s);
+void gather_out(int dtype, const void *state, double *staging, uint64_t offset, uint64_t count,
+                uint32_t C, hipStream_t s);
+void scale_block(int dtype, void *state, uint64_t j, uint64_t D, double factor, hipStream_t s);
+
+// device-computed determinant pairs of one term (for bit-exact index tests)
+void debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *idx01, uint32_t *idx10,
+                 uint32_t *parity, uint64_t count, hipStream_t s);
+
+int device_max_lds();
+
+}  // namespace qpe

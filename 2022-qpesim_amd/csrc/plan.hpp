@@ -1,0 +1,110 @@
+// Evolution plan: one Trotter step U (the reference's applyUnitary_reordered,
+// cpp/time_evolution_over_terms.hpp:374-562) compiled into a short list of
+// tile passes over one block [alpha][beta] of the register.
+//
+// Block layout in HBM: column-major within a block, amplitude (a, b) of block
+// j at j*C*C + b*C + a (a = alpha rank, b = beta rank).  A "column" (fixed
+// beta string, all alpha strings) is contiguous.
+//
+// Pass kinds
+//   COL  a workgroup stages a set of whole columns in LDS and applies, in
+//        reference order, the diagonal phase, the alpha excitation groups, the
+//        pure-alpha double excitations and mixed alpha-beta double excitations
+//        whose beta orbitals lie in the pass's orbit set T (the columns a
+//        workgroup holds are closed under those terms).
+//   ROW  a workgroup stages a set of whole rows (fixed alpha string) and
+//        applies beta excitation groups or pure-beta double excitations.
+//
+// Exact fusions (mathematically identical to the reference's sequence):
+//   * all pp and pqqp terms commute: one diagonal phase per amplitude;
+//   * a pq term and the pqqr terms applied right after it (TEOT:428-513) act
+//     on the same determinant pairs with the same Jordan-Wigner sign and commute,
+//     so their product is one rotation whose angle is t*(h_pq + sum of the
+//     pqqr coefficients whose inner orbital is occupied).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hamiltonian.hpp"
+
+namespace qpe {
+
+enum OpType : int32_t {
+    OP_DIAG = 0,   // diagonal phase (all pp + pqqp)
+    OP_GA = 1,     // alpha excitation group (pq + gated pqqr)
+    OP_GB = 2,     // beta excitation group
+    OP_AAAA = 3,   // pqrs, all alpha
+    OP_BBBB = 4,   // pqrs, all beta
+    OP_ABBA = 5    // pqrs, alpha p,s / beta q,r
+};
+
+enum PassKind : int32_t { PASS_COL = 0, PASS_ROW = 1 };
+
+enum PassFlags : int32_t {
+    PF_FROM_PREV = 1,  // read block j-1 (scaled by its renormalisation factor), write block j
+    PF_EMIT_NORM = 2   // write per-workgroup FP64 partial sums of |a|^2
+};
+
+struct OpDev {
+    int32_t type;
+    int32_t index;   // group index (GA/GB) or term index (pqrs)
+};
+
+struct GroupDev {
+    int32_t x, y;    // orbitals of the excitation inside its spin string (x > y)
+    int32_t spin;    // 0 alpha, 1 beta
+    int32_t pad;
+};
+
+struct TermDev {
+    int32_t p, q, r, s;   // local orbital indices (alpha p,s / beta q,r for ABBA)
+    double cs, sn;        // cos(c t), sin(c t)
+};
+
+struct PassDev {
+    int32_t kind;
+    int32_t flags;
+    int32_t op_begin, op_end;
+    int32_t item_begin, item_count;
+    int32_t max_lines;    // columns (COL) or rows (ROW) per workgroup
+    int32_t pad;
+};
+
+struct ItemDev {
+    uint32_t begin;       // COL: offset into the column list; ROW: first row
+    uint32_t count;       // number of columns / rows
+};
+
+struct Plan {
+    int n = 0, e = 0;
+    uint32_t C = 0;
+    uint32_t ca_pairs = 0;     // C(n-2, e-1)
+    uint32_t ccaa_pairs = 0;   // C(n-4, e-2)
+    size_t elem_bytes = 16;
+    size_t tile_bytes = 0;     // LDS bytes per workgroup for the amplitude tile
+
+    std::vector<uint32_t> strings;      // [C] the colex-ordered strings of one spin
+    std::vector<double> theta_a, theta_b, cross;   // diag: [C], [C], [n][C]
+    std::vector<GroupDev> groups;
+    std::vector<double> grp_a;          // [G][ca_pairs][2] cos, sin of the pair angle part
+    std::vector<double> grp_b;          // [G][C][2] cos, sin of the other-string angle part
+    std::vector<TermDev> terms;
+    std::vector<OpDev> ops;
+    std::vector<PassDev> passes;
+    std::vector<ItemDev> items;
+    std::vector<uint32_t> col_list;     // column ranks of COL items
+
+    // work accounting per U
+    uint64_t pair_updates = 0;   // determinant-pair rotations of the reference's terms
+    uint64_t diag_updates = 0;   // diagonal multiplies of the reference's terms
+    uint64_t tile_bytes_moved = 0;   // HBM bytes the passes load + store
+    int max_items = 0;
+};
+
+// Builds the plan.  tile_bytes: LDS budget for the amplitude tile.  Throws
+// std::runtime_error with a message on unsupported sizes.
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes);
+
+}  // namespace qpe

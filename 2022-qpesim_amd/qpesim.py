@@ -1,0 +1,371 @@
+"""Python binding of libqpesim.so (ctypes over include/qpesim.h) and the
+seeded synthetic-Hamiltonian generator used by the tests and the benchmark.
+
+The binding mirrors the reference call sequence (christopherkang/2022-qpesim,
+cpp/main.cc:99-134):
+
+    h = Hamiltonian(); <add integrals>; h.index_terms(); h.set_timestep(tau)
+    sv = StateVector(m, n_orb, n_occ, dtype); sv.set(0, psi0); sv.scale_block(0, 1/sqrt(2^m))
+    time_evolve_over_terms(sv, h); sv.qift(); probs = sv.identify_phase_amplitudes()
+
+There is no CPU fallback: if the library is missing this module raises.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libqpesim.so")
+
+COMPLEX128, COMPLEX64 = 0, 1
+PP, PQ, PQQP, PQQR, PQRS = range(5)
+OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES = 0, 1, 2
+ERR_ARG, ERR_OOM, ERR_HIP, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
+
+# every entry point include/qpesim.h declares
+EXPORTS = [
+    "qpe_hamiltonian_create", "qpe_hamiltonian_destroy", "qpe_hamiltonian_add_one_body_term",
+    "qpe_hamiltonian_add_two_body_term", "qpe_hamiltonian_set_num_orbitals", "qpe_hamiltonian_set_num_occupied",
+    "qpe_hamiltonian_index_terms", "qpe_hamiltonian_set_timestep", "qpe_hamiltonian_term_counts",
+    "qpe_hamiltonian_get_terms", "qpe_state_create", "qpe_state_destroy", "qpe_state_size", "qpe_state_set",
+    "qpe_state_get", "qpe_state_scale_block", "qpe_time_evolve_over_terms", "qpe_qift",
+    "qpe_identify_phase_amplitudes", "qpe_state_set_option", "qpe_state_timings", "qpe_state_block_norms",
+    "qpe_debug_pairs", "qpe_last_error",
+]
+
+_lib = None
+
+
+class QpeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qpesim error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libqpesim.so (built by `make -C 2022-qpesim_amd`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, i64, dbl, ci = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    P = ctypes.POINTER
+    sig = {
+        "qpe_hamiltonian_create": [P(vp)], "qpe_hamiltonian_destroy": [vp],
+        "qpe_hamiltonian_add_one_body_term": [vp, ci, ci, dbl],
+        "qpe_hamiltonian_add_two_body_term": [vp, ci, ci, ci, ci, dbl],
+        "qpe_hamiltonian_set_num_orbitals": [vp, ctypes.c_uint], "qpe_hamiltonian_set_num_occupied": [vp, ctypes.c_uint],
+        "qpe_hamiltonian_index_terms": [vp], "qpe_hamiltonian_set_timestep": [vp, dbl],
+        "qpe_hamiltonian_term_counts": [vp, P(u64)],
+        "qpe_hamiltonian_get_terms": [vp, ci, P(ctypes.c_int32), P(dbl), u64],
+        "qpe_state_create": [P(vp), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ci],
+        "qpe_state_destroy": [vp], "qpe_state_size": [vp, P(u64)],
+        "qpe_state_set": [vp, u64, u64, P(dbl)], "qpe_state_get": [vp, u64, u64, P(dbl)],
+        "qpe_state_scale_block": [vp, u64, dbl], "qpe_time_evolve_over_terms": [vp, vp], "qpe_qift": [vp],
+        "qpe_identify_phase_amplitudes": [vp, P(dbl)], "qpe_state_set_option": [vp, ci, i64],
+        "qpe_state_timings": [vp, P(dbl)], "qpe_state_block_norms": [vp, P(dbl)],
+        "qpe_debug_pairs": [ci, ci, ci, P(ctypes.c_int32), P(ctypes.c_uint32), P(ctypes.c_uint32),
+                            P(ctypes.c_uint32), u64, P(u64)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ci
+    L.qpe_last_error.argtypes = []
+    L.qpe_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _call(name, *args):
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        raise QpeError(rc, L.qpe_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Hamiltonian:
+    """qpe_hamiltonian_t: the reference's Hamiltonian (cpp/hamiltonian.hpp:24-184)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _call("qpe_hamiltonian_create", ctypes.byref(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None and self._h.value:
+            _lib.qpe_hamiltonian_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def add_one_body_term(self, p, q, coeff):
+        _call("qpe_hamiltonian_add_one_body_term", self._h, int(p), int(q), float(coeff))
+
+    def add_two_body_term(self, p, q, r, s, coeff):
+        _call("qpe_hamiltonian_add_two_body_term", self._h, int(p), int(q), int(r), int(s), float(coeff))
+
+    def set_num_orbitals(self, n):
+        _call("qpe_hamiltonian_set_num_orbitals", self._h, int(n))
+
+    def set_num_occupied(self, e):
+        _call("qpe_hamiltonian_set_num_occupied", self._h, int(e))
+
+    def index_terms(self):
+        _call("qpe_hamiltonian_index_terms", self._h)
+
+    def set_timestep(self, t):
+        _call("qpe_hamiltonian_set_timestep", self._h, float(t))
+
+    def term_counts(self):
+        c = (ctypes.c_uint64 * 5)()
+        _call("qpe_hamiltonian_term_counts", self._h, c)
+        return [int(x) for x in c]
+
+    def terms(self, cls):
+        n = self.term_counts()[cls]
+        idx = np.zeros((max(n, 1), 4), np.int32)
+        co = np.zeros(max(n, 1), np.float64)
+        _call("qpe_hamiltonian_get_terms", self._h, int(cls), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+              _dptr(co), max(n, 1))
+        return idx[:n], co[:n]
+
+    def sum_abs_coeff(self):
+        return float(sum(np.abs(self.terms(c)[1]).sum() for c in range(5)))
+
+    @property
+    def handle(self):
+        return self._h
+
+
+class StateVector:
+    """qpe_state_t: the reference's StateVector (cpp/state_vector.hpp:16-42) resident in HBM."""
+
+    def __init__(self, precision, num_orbitals, num_occupied, dtype=COMPLEX128):
+        self._s = ctypes.c_void_p()
+        _call("qpe_state_create", ctypes.byref(self._s), int(precision), int(num_orbitals), int(num_occupied), int(dtype))
+        self.m = int(precision)
+        self.M = 1 << self.m
+        self.n = int(num_orbitals)
+        self.e = int(num_occupied)
+        self.C = math.comb(self.n, self.e)
+        self.D = self.C * self.C
+        self.dtype = dtype
+
+    def close(self):
+        if getattr(self, "_s", None) and _lib is not None and self._s.value:
+            _lib.qpe_state_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def size(self):
+        n = ctypes.c_uint64()
+        _call("qpe_state_size", self._s, ctypes.byref(n))
+        return int(n.value)
+
+    def set(self, offset, amps):
+        a = np.ascontiguousarray(amps, dtype=np.complex128).ravel()
+        _call("qpe_state_set", self._s, int(offset), a.size, _dptr(a))
+
+    def get(self, offset=0, count=None):
+        if count is None:
+            count = self.size() - offset
+        out = np.empty(int(count), np.complex128)
+        _call("qpe_state_get", self._s, int(offset), int(count), _dptr(out))
+        return out
+
+    def scale_block(self, block, factor):
+        _call("qpe_state_scale_block", self._s, int(block), float(factor))
+
+    def set_option(self, opt, value):
+        _call("qpe_state_set_option", self._s, int(opt), int(value))
+
+    def qift(self):
+        _call("qpe_qift", self._s)
+
+    def identify_phase_amplitudes(self):
+        out = np.zeros(self.M, np.float64)
+        _call("qpe_identify_phase_amplitudes", self._s, _dptr(out))
+        return out
+
+    def timings(self):
+        out = np.zeros(5, np.float64)
+        _call("qpe_state_timings", self._s, _dptr(out))
+        return {"evolve_ms": out[0], "qift_ms": out[1], "pk_ms": out[2], "passes_per_step": int(out[3]),
+                "launches_per_step": int(out[4])}
+
+    def block_norms(self):
+        out = np.zeros(self.M, np.float64)
+        _call("qpe_state_block_norms", self._s, _dptr(out))
+        return out
+
+    @property
+    def handle(self):
+        return self._s
+
+
+def time_evolve_over_terms(sv, h):
+    """time_evolve_over_terms(sv, h, m), cpp/time_evolution_over_terms.hpp:92-139."""
+    _call("qpe_time_evolve_over_terms", sv.handle, h.handle)
+
+
+def debug_pairs(kind, n, e, args):
+    """Device-computed (idx01, idx10, parity) of every pair of one term."""
+    a = np.zeros(4, np.int32)
+    a[:len(args)] = args
+    cap = math.comb(n, max(0, e)) + 1
+    o1 = np.zeros(cap, np.uint32)
+    o2 = np.zeros(cap, np.uint32)
+    op = np.zeros(cap, np.uint32)
+    cnt = ctypes.c_uint64()
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    _call("qpe_debug_pairs", int(kind), int(n), int(e), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+          o1.ctypes.data_as(u32p), o2.ctypes.data_as(u32p), op.ctypes.data_as(u32p), cap, ctypes.byref(cnt))
+    k = int(cnt.value)
+    return o1[:k], o2[:k], op[:k]
+
+
+# --------------------------------------------------------------------------
+# seeded synthetic Hamiltonians
+# --------------------------------------------------------------------------
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, count, start=0):
+    """Outputs start..start+count-1 of the SplitMix64 stream seeded with `seed`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def normals(seed, count, start=0):
+    """Standard normals by Box-Muller over pairs of SplitMix64 uniforms;
+    normal 2i uses the cosine branch and 2i+1 the sine branch of pair i."""
+    npairs = (count + 1) // 2
+    z = splitmix64(seed, 2 * npairs, 2 * (start // 2))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u1 = 1.0 - u[0::2]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(2 * npairs)
+    out[0::2] = r * np.cos(2.0 * np.pi * u2)
+    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    off = start % 2
+    return out[off:off + count]
+
+
+CUTOFF = 1e-8
+
+
+def synthetic_integrals(n_spatial, seed):
+    """Broombridge-style integral lists for the synthetic family:
+
+    one-body  h_pp = -2.0 + 0.25 p,  h_pq ~ N(0, 0.05^2) (p > q, symmetric)
+    two-body  (pq|rs) = N(0, 0.1^2) * exp(-0.3 (|p-q| + |r-s|)) on the 8-fold-unique
+              set p >= q, r >= s, pq >= rs (compound index), plus 0.5/(1+|p-r|)
+              on (pp|rr);  |value| < 1e-8 dropped.
+    Orbitals are 0-based in the formulas; entries are 1-based like Broombridge.
+    Normals are drawn in this order: one-body off-diagonals (p ascending, q < p
+    ascending), then two-body values in (pq, rs) compound order.
+    Returns (one_body [(p, q, v)], two_body [(a, b, c, d, v)] Mulliken (ab|cd))."""
+    n = n_spatial
+    pairs = [(p, q) for p in range(n) for q in range(p + 1)]
+    n_off = n * (n - 1) // 2
+    n_two = len(pairs) * (len(pairs) + 1) // 2
+    g = normals(seed, n_off + n_two)
+    one, k = [], 0
+    for p in range(n):
+        for q in range(p + 1):
+            if p == q:
+                v = -2.0 + 0.25 * p
+            else:
+                v = 0.05 * g[k]
+                k += 1
+            if abs(v) >= CUTOFF:
+                one.append((p + 1, q + 1, float(v)))
+    two = []
+    for i, (p, q) in enumerate(pairs):
+        for j in range(i + 1):
+            r, s = pairs[j]
+            v = 0.1 * g[k] * math.exp(-0.3 * (abs(p - q) + abs(r - s)))
+            k += 1
+            if p == q and r == s:
+                v += 0.5 / (1.0 + abs(p - r))
+            if abs(v) >= CUTOFF:
+                two.append((p + 1, q + 1, r + 1, s + 1, float(v)))
+    return one, two
+
+
+def build_hamiltonian(n_spatial, n_occupied, one_body, two_body):
+    """Feeds Broombridge entries as IngestBroombridge does (cpp/hamiltonian.cc:22-52):
+    one-body [p,q,v] swapped to p >= q; two-body [a,b,c,d,v] read as i=a, l=b,
+    j=c, k=d and passed as AddTwoBodyTerm(i, j, k, l).  Then IndexTerms and
+    tau = 0.5 / sum |coefficient| over the indexed terms."""
+    h = Hamiltonian()
+    h.set_num_orbitals(n_spatial)
+    h.set_num_occupied(n_occupied)
+    for p, q, v in one_body:
+        if q > p:
+            p, q = q, p
+        h.add_one_body_term(p, q, v)
+    for a, b, c, d, v in two_body:
+        h.add_two_body_term(a, c, d, b, v)
+    h.index_terms()
+    tau = 0.5 / h.sum_abs_coeff()
+    h.set_timestep(tau)
+    return h, tau
+
+
+def initial_state(n_spatial, n_occupied, seed=None, hf_weight=1.0):
+    """Block-0 amplitudes: Hartree-Fock (rank 0 in both strings, amplitude +1) or
+    normalize(hf_weight*HF + (1-hf_weight)*r), r a unit complex Gaussian vector
+    over all C(n,e)^2 determinants drawn from the SplitMix64 stream seed+7919."""
+    C = math.comb(n_spatial, n_occupied)
+    psi = np.zeros(C * C, np.complex128)
+    psi[0] = 1.0
+    if hf_weight < 1.0:
+        g = normals(seed + 7919, 2 * C * C)
+        r = g[0::2] + 1j * g[1::2]
+        r /= np.linalg.norm(r)
+        psi = hf_weight * psi + (1.0 - hf_weight) * r
+        psi /= np.linalg.norm(psi)
+    return psi
+
+
+# the workloads, by name: spatial orbitals, electrons per spin, ancillas, seed, HF weight
+CONFIGS = {
+    "cfg0": dict(n=6, e=3, m=6, seed=0, hf=1.0),        # n=12 spin-orbitals, N=6
+    "cfg1": dict(n=14, e=5, m=8, seed=1, hf=0.9),       # n=28, N=10
+    "cfg2": dict(n=16, e=4, m=10, seed=2, hf=1.0),      # n=32, N=8
+    "cfg4": dict(n=17, e=5, m=7, seed=4, hf=0.95),      # n=34, N=10
+}
+
+
+def batch_instances(count=48, seed=3):
+    """48 instances: spin-orbital count n uniform on {12,14,...,24}, electron count
+    N uniform on {4,6,8} with N <= n/2 (redrawn otherwise); instance i uses seed 1000+i."""
+    z = splitmix64(seed, 4 * count)
+    out, k = [], 0
+    while len(out) < count:
+        n = 12 + 2 * int(z[k % len(z)] % np.uint64(7))
+        N = 4 + 2 * int(z[(k + 1) % len(z)] % np.uint64(3))
+        k += 2
+        if k >= len(z):
+            z = splitmix64(seed + k, 4 * count)
+            k = 0
+        if N <= n // 2:
+            out.append(dict(n=n // 2, e=N // 2, m=8, seed=1000 + len(out), hf=1.0))
+    return out

@@ -1,0 +1,151 @@
+/*
+ * qpesim.h -- C ABI of the MI355X-native QPE simulator core (libqpesim.so).
+ *
+ * The entry points stand in for the reference call sequence of
+ * christopherkang/2022-qpesim, cpp/main.cc:99-134:
+ *
+ *   Hamiltonian h; h.AddOneBodyTerm/AddTwoBodyTerm (via IngestBroombridge);
+ *   h.IndexTerms(HalfUp); h.SetTimestep(t);
+ *   StateVector sv(m, n_orb, n_occ); sv[i] = ...; sv[0..C^2) *= 1/sqrt(2^m);
+ *   time_evolve_over_terms(sv, h, m); sv.qift(); probs = sv.IdentifyPhaseAmplitudes();
+ *
+ * Plain C types only.  Every call returns QPE_OK (0) or a QPE_ERR_* code; the
+ * message of the last failure on the calling thread is qpe_last_error().
+ * A handle is used by one host thread at a time; each state handle owns one
+ * HIP stream, and calls on it are stream-ordered.  Amplitudes cross the ABI as
+ * interleaved (re, im) doubles in the reference's index order
+ * j*C^2 + rank_alpha*C + rank_beta (cpp/time_evolution_over_terms.hpp:169-171).
+ */
+#ifndef QPESIM_H
+#define QPESIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum qpe_status {
+    QPE_OK = 0,
+    QPE_ERR_ARG = 1,      /* invalid handle or argument */
+    QPE_ERR_OOM = 2,      /* request larger than the free device memory */
+    QPE_ERR_HIP = 3,      /* a HIP runtime call failed */
+    QPE_ERR_NUMERIC = 4,  /* non-finite amplitudes detected */
+    QPE_ERR_STATE = 5     /* call out of sequence (e.g. evolve before IndexTerms) */
+};
+
+enum qpe_dtype {
+    QPE_COMPLEX128 = 0,   /* FP64 state and rotations (reference precision) */
+    QPE_COMPLEX64 = 1     /* FP32 state and rotations; FP64 angles, norms, P(k) */
+};
+
+/* term classes, in the order PrintIndexedSegmentedHamiltonianTerms returns them */
+enum qpe_term_class { QPE_PP = 0, QPE_PQ = 1, QPE_PQQP = 2, QPE_PQQR = 3, QPE_PQRS = 4 };
+
+typedef struct qpe_hamiltonian *qpe_hamiltonian_t;
+typedef struct qpe_state *qpe_state_t;
+
+/* ---------------- Hamiltonian (term compiler, host) ---------------- */
+
+/* Hamiltonian::Hamiltonian(), cpp/hamiltonian.hpp:27 */
+int qpe_hamiltonian_create(qpe_hamiltonian_t *out);
+int qpe_hamiltonian_destroy(qpe_hamiltonian_t h);
+
+/* Hamiltonian::AddOneBodyTerm, cpp/hamiltonian.cc:301-323.  p, q are 1-based
+ * spatial orbitals with p >= q (IngestBroombridge swaps, cpp/hamiltonian.cc:29-32). */
+int qpe_hamiltonian_add_one_body_term(qpe_hamiltonian_t h, int p, int q, double coeff);
+
+/* Hamiltonian::AddTwoBodyTerm, cpp/hamiltonian.cc:325-398: term a+_p a+_q a_r a_s
+ * over 1-based spatial orbitals; a Broombridge Mulliken entry [a,b,c,d] is
+ * passed as (p,q,r,s) = (a,c,d,b) (cpp/hamiltonian.cc:43-48). */
+int qpe_hamiltonian_add_two_body_term(qpe_hamiltonian_t h, int p, int q, int r, int s, double coeff);
+
+/* IngestBroombridge fills these from the file (cpp/hamiltonian.cc:34-35,113-115);
+ * Hamiltonian::SetNumOccupied, cpp/hamiltonian.hpp:72. */
+int qpe_hamiltonian_set_num_orbitals(qpe_hamiltonian_t h, unsigned num_orbitals);
+int qpe_hamiltonian_set_num_occupied(qpe_hamiltonian_t h, unsigned num_occupied);
+
+/* Hamiltonian::IndexTerms(SpinIndex::HalfUp), cpp/hamiltonian.cc:160-197 */
+int qpe_hamiltonian_index_terms(qpe_hamiltonian_t h);
+
+/* Hamiltonian::SetTimestep, cpp/hamiltonian.hpp:71 */
+int qpe_hamiltonian_set_timestep(qpe_hamiltonian_t h, double t);
+
+/* sizes of the five lists PrintIndexedSegmentedHamiltonianTerms returns,
+ * cpp/hamiltonian.cc:253-299 */
+int qpe_hamiltonian_term_counts(qpe_hamiltonian_t h, uint64_t counts[5]);
+
+/* Copies list `cls` (map order): idx[4*i..4*i+3] HalfUp indices (-1 pads the
+ * one-body lists), coeff[i] the accumulated coefficient.  cap = entries. */
+int qpe_hamiltonian_get_terms(qpe_hamiltonian_t h, int cls, int32_t *idx, double *coeff, uint64_t cap);
+
+/* ---------------- StateVector (device) ---------------- */
+
+/* StateVector(precision, num_orbitals, num_occupied), cpp/state_vector.hpp:20-42:
+ * a zero-filled [2^m][C(n,e)][C(n,e)] register, here resident in HBM. */
+int qpe_state_create(qpe_state_t *out, unsigned precision, unsigned num_orbitals,
+                     unsigned num_occupied, int dtype);
+int qpe_state_destroy(qpe_state_t s);
+
+/* number of amplitudes 2^m * C(n,e)^2 */
+int qpe_state_size(qpe_state_t s, uint64_t *n_amplitudes);
+
+/* StateVector::operator[] writes/reads, batched: amplitudes [offset, offset+count)
+ * in reference index order, interleaved (re, im) doubles.  Host memory is
+ * borrowed for the duration of the call. */
+int qpe_state_set(qpe_state_t s, uint64_t offset, uint64_t count, const double *amplitudes);
+int qpe_state_get(qpe_state_t s, uint64_t offset, uint64_t count, double *amplitudes);
+
+/* sv[j*C^2 + i] *= factor for all i: the implicit Hadamards of cpp/main.cc:111-115 */
+int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor);
+
+/* ---------------- the hot path ---------------- */
+
+/* time_evolve_over_terms(sv, h, m), cpp/time_evolution_over_terms.hpp:92-139:
+ * block j = U * block(j-1) for j = 1..2^m-1, U one first-order Trotter step in
+ * the reference's term order. */
+int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h);
+
+/* StateVector::qift(), cpp/state_vector.hpp:115-179: in place, forward DFT
+ * (e^{-2 pi i jk/M}) along the ancilla axis, natural order, divided by sqrt(M). */
+int qpe_qift(qpe_state_t s);
+
+/* StateVector::IdentifyPhaseAmplitudes(), cpp/state_vector.hpp:341-372:
+ * probs[k] = sum_d |y_k(d)|^2, k < 2^m (FP64, compensated sums). */
+int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs);
+
+/* ---------------- options, diagnostics ---------------- */
+
+enum qpe_option {
+    QPE_OPT_RENORMALIZE = 0,   /* 1: renormalise every block to block 0's norm (default: 1 for
+                                  complex64, 0 for complex128) */
+    QPE_OPT_ENGINE = 1,        /* 0: fused tile engine (default); 1: one launch per term */
+    QPE_OPT_MAX_TILE_BYTES = 2 /* LDS bytes a tile may use (0: device maximum) */
+};
+int qpe_state_set_option(qpe_state_t s, int option, int64_t value);
+
+/* timings of the last calls in milliseconds (device events):
+ * out[0] evolve, out[1] qift, out[2] P(k); out[3] passes per U; out[4] launches per U */
+int qpe_state_timings(qpe_state_t s, double out[5]);
+
+/* per-block squared norms measured during the last evolve (FP64, compensated);
+ * norms[j] for j < 2^m, before renormalisation. */
+int qpe_state_block_norms(qpe_state_t s, double *norms);
+
+/* Determinant pairs of one term as the device computes them (rank/unrank and
+ * Jordan-Wigner parity on the GPU): kind 2 = CA(p, r), kind 4 = CCAA(p,q,r,s),
+ * kind 3 = CNA(p,q,r) in one spin string of n orbitals with e electrons,
+ * argument order as discoverCA/CNA/CCAA_BVs (cpp/time_evolution_over_terms.hpp:
+ * 1160-1324).  Writes idx01[i], idx10[i], parity[i] for every pair; *count
+ * receives the number of pairs. */
+int qpe_debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *idx01,
+                    uint32_t *idx10, uint32_t *parity, uint64_t cap, uint64_t *count);
+
+/* message of the last failed call on this thread ("" if none) */
+const char *qpe_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QPESIM_H */
