@@ -481,6 +481,30 @@ int qpe_state_block_norms(qpe_state_t s, double *norms) {
     });
 }
 
+int qpe_plan_info(qpe_hamiltonian_t h, int dtype, uint64_t tile_bytes, double out[8]) {
+    return guarded([&] {
+        check_ham(h);
+        require(out != nullptr, QPE_ERR_ARG, "null output");
+        require(dtype == QPE_COMPLEX128 || dtype == QPE_COMPLEX64, QPE_ERR_ARG, "unknown dtype");
+        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+        size_t budget = tile_bytes ? (size_t)tile_bytes : (size_t)(160 * 1024 - 4096);
+        qpe::Plan p;
+        try {
+            p = qpe::build_plan(h->h, dtype == QPE_COMPLEX128 ? 16 : 8, budget);
+        } catch (const std::runtime_error &e) {
+            throw ApiError(QPE_ERR_ARG, e.what());
+        }
+        out[0] = (double)p.passes.size();
+        out[1] = (double)p.passes.size() + 1;
+        out[2] = (double)p.pair_updates;
+        out[3] = (double)p.diag_updates;
+        out[4] = (double)p.tile_bytes_moved;
+        out[5] = (double)p.C;
+        out[6] = (double)p.groups.size();
+        out[7] = (double)p.terms.size();
+    });
+}
+
 int qpe_debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *idx01, uint32_t *idx10, uint32_t *parity,
                     uint64_t cap, uint64_t *count) {
     return guarded([&] {

@@ -32,7 +32,7 @@ EXPORTS = [
     "qpe_hamiltonian_get_terms", "qpe_state_create", "qpe_state_destroy", "qpe_state_size", "qpe_state_set",
     "qpe_state_get", "qpe_state_scale_block", "qpe_time_evolve_over_terms", "qpe_qift",
     "qpe_identify_phase_amplitudes", "qpe_state_set_option", "qpe_state_timings", "qpe_state_block_norms",
-    "qpe_debug_pairs", "qpe_last_error",
+    "qpe_debug_pairs", "qpe_plan_info", "qpe_last_error",
 ]
 
 _lib = None
@@ -70,6 +70,7 @@ def load():
         "qpe_state_timings": [vp, P(dbl)], "qpe_state_block_norms": [vp, P(dbl)],
         "qpe_debug_pairs": [ci, ci, ci, P(ctypes.c_int32), P(ctypes.c_uint32), P(ctypes.c_uint32),
                             P(ctypes.c_uint32), u64, P(u64)],
+        "qpe_plan_info": [vp, ci, u64, P(dbl)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -216,6 +217,14 @@ def time_evolve_over_terms(sv, h):
     _call("qpe_time_evolve_over_terms", sv.handle, h.handle)
 
 
+def plan_info(h, dtype=COMPLEX128, tile_bytes=0):
+    """Host-only summary of the tile program of one Trotter step."""
+    out = np.zeros(8, np.float64)
+    _call("qpe_plan_info", h.handle, int(dtype), int(tile_bytes), _dptr(out))
+    keys = ["passes", "launches", "pair_updates", "diag_updates", "pass_bytes", "C", "groups", "double_terms"]
+    return {k: (int(v) if k != "pass_bytes" else float(v)) for k, v in zip(keys, out)}
+
+
 def debug_pairs(kind, n, e, args):
     """Device-computed (idx01, idx10, parity) of every pair of one term."""
     a = np.zeros(4, np.int32)
@@ -254,7 +263,8 @@ def splitmix64(seed, count, start=0):
 def normals(seed, count, start=0):
     """Standard normals by Box-Muller over pairs of SplitMix64 uniforms;
     normal 2i uses the cosine branch and 2i+1 the sine branch of pair i."""
-    npairs = (count + 1) // 2
+    off = start % 2
+    npairs = (off + count + 1) // 2
     z = splitmix64(seed, 2 * npairs, 2 * (start // 2))
     u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     u1 = 1.0 - u[0::2]
@@ -263,7 +273,6 @@ def normals(seed, count, start=0):
     out = np.empty(2 * npairs)
     out[0::2] = r * np.cos(2.0 * np.pi * u2)
     out[1::2] = r * np.sin(2.0 * np.pi * u2)
-    off = start % 2
     return out[off:off + count]
 
 

@@ -142,6 +142,14 @@ int qpe_state_block_norms(qpe_state_t s, double *norms);
 int qpe_debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *idx01,
                     uint32_t *idx10, uint32_t *parity, uint64_t cap, uint64_t *count);
 
+/* Host-only description of the tile program one Trotter step compiles to, for
+ * a state of this dtype and an LDS tile budget (0: 160 KiB - 4 KiB):
+ * out[0] passes per step, out[1] kernel launches per step, out[2] determinant
+ * pair rotations per step of the reference's terms, out[3] diagonal updates per
+ * step, out[4] HBM bytes the passes load+store per step, out[5] C(n,e),
+ * out[6] fused excitation groups, out[7] double-excitation terms. */
+int qpe_plan_info(qpe_hamiltonian_t h, int dtype, uint64_t tile_bytes, double out[8]);
+
 /* message of the last failed call on this thread ("" if none) */
 const char *qpe_last_error(void);
 
