@@ -25,7 +25,7 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 1024;
 constexpr int kMaxLines = 64;
 constexpr int kBinomWords = kBinomRows * kBinomCols;
 
@@ -67,6 +67,7 @@ struct PassArgs {
     const OpDev *ops;
     const ItemDev *items;
     const uint32_t *col_list;
+    const uint32_t *ca_table, *ccaa_table;
     double *partials;
     const double *scale;
     int n, e;
@@ -74,11 +75,194 @@ struct PassArgs {
     int renorm;
 };
 
-// Number of lanes that share one determinant pair (power of two, <= lines).
-__device__ inline int lanes_per_pair(int npairs, int nlines) {
-    int lp = 1;
-    while (lp * 2 <= nlines && npairs * lp * 2 <= kThreads) lp *= 2;
-    return lp;
+// Barrier for LDS traffic only: outstanding global loads (the prefetched pair
+// entries of later ops) stay in flight across it.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Work split of one op: a thread owns up to two determinant pairs (k0, k1) and
+// the lines lane, lane + L, ...  (L threads share one pair).
+struct Split {
+    int L, lane, k0, k1;
+};
+
+__device__ inline Split split_for(int np) {
+    Split sp;
+    const int t = threadIdx.x;
+    if (np <= 0) {
+        sp.L = 1; sp.lane = 0; sp.k0 = -1; sp.k1 = -1;
+    } else if (np <= kThreads) {
+        sp.L = kThreads / np;
+        sp.k0 = t < sp.L * np ? t % np : -1;
+        sp.lane = t < sp.L * np ? t / np : 0;
+        sp.k1 = -1;
+    } else {
+        sp.L = 1; sp.lane = 0;
+        sp.k0 = t;
+        sp.k1 = t + kThreads < np ? t + kThreads : -1;
+    }
+    return sp;
+}
+
+// Everything one op needs that does not depend on amplitudes; fetched
+// kPrefetch ops ahead so that its global-memory latency overlaps earlier ops.
+struct OpPre {
+    int type, index;
+    uint32_t aux;
+    int np;
+    uint32_t e0, e1;
+    double a0c, a0s, a1c, a1s;
+    double cs, sn;
+};
+
+__device__ inline void fetch_op(const PassArgs &A, int o, int o_end, OpPre &P) {
+    P.type = -1;
+    if (o >= o_end) return;
+    const OpDev od = A.ops[o];
+    P.type = od.type;
+    P.index = od.index;
+    P.aux = od.aux;
+    const uint32_t *tab = nullptr;
+    P.np = 0;
+    P.cs = 1.0; P.sn = 0.0;
+    if (od.type == OP_AAAA || od.type == OP_BBBB) {
+        P.np = (int)A.ccaa_pairs;
+        tab = A.ccaa_table + od.tab;
+    } else if (od.type != OP_DIAG) {
+        P.np = (int)A.ca_pairs;
+        tab = A.ca_table + od.tab;
+    }
+    if (od.type >= OP_AAAA) {
+        P.cs = A.terms[od.index].cs;
+        P.sn = A.terms[od.index].sn;
+    }
+    const Split sp = split_for(P.np);
+    P.e0 = sp.k0 >= 0 ? tab[sp.k0] : 0u;
+    P.e1 = sp.k1 >= 0 ? tab[sp.k1] : 0u;
+    P.a0c = 1.0; P.a0s = 0.0; P.a1c = 1.0; P.a1s = 0.0;
+    if (od.type == OP_GA || od.type == OP_GB) {
+        const double *ga = A.grp_a + 2ull * od.index * A.ca_pairs;
+        if (sp.k0 >= 0) { P.a0c = ga[2 * sp.k0]; P.a0s = ga[2 * sp.k0 + 1]; }
+        if (sp.k1 >= 0) { P.a1c = ga[2 * sp.k1]; P.a1s = ga[2 * sp.k1 + 1]; }
+    }
+}
+
+__device__ inline uint32_t rank1(uint32_t e) { return e & kRankMask; }
+__device__ inline uint32_t rank2(uint32_t e) { return (e >> kRankBits) & kRankMask; }
+__device__ inline uint32_t par_of(uint32_t e) { return e >> 31; }
+
+template <typename R>
+struct Tile {
+    typename Vec<R>::type *v;      // [nl][C]
+    double2 *gang;                 // staged group angles [group slot][nl]
+    const uint32_t *line_rank, *line_str;
+    int *pair_list, *pair_count;
+    int nl;
+    uint32_t C;
+};
+
+template <typename R>
+__device__ inline void apply_op(const PassArgs &A, Tile<R> &T, const OpPre &P, uint32_t &cur_qr) {
+    using V = typename Vec<R>::type;
+    if (P.type < 0) return;
+    lds_barrier();
+    const int nl = T.nl;
+    const uint32_t C = T.C;
+    V *tile = T.v;
+    if (P.type == OP_DIAG) {
+        const uint32_t total = (uint32_t)nl * C;
+        for (uint32_t idx = threadIdx.x; idx < total; idx += kThreads) {
+            uint32_t l = idx / C, a = idx - l * C;
+            uint32_t b = T.line_rank[l];
+            uint32_t sa = A.strings[a];
+            double th = A.theta_a[a] + A.theta_b[b];
+            while (sa) {
+                int p = __builtin_ctz(sa);
+                th += A.cross[(uint64_t)p * C + b];
+                sa &= sa - 1;
+            }
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            V v = tile[idx], w;
+            R c = (R)cs, s = (R)(-sn);
+            w.x = c * v.x - s * v.y;
+            w.y = c * v.y + s * v.x;
+            tile[idx] = w;
+        }
+        return;
+    }
+    const Split sp = split_for(P.np);
+    if (P.type == OP_GA || P.type == OP_GB) {
+        const double2 *gang = T.gang + (size_t)P.aux * nl;
+        for (int h = 0; h < 2; ++h) {
+            int k = h ? sp.k1 : sp.k0;
+            if (k < 0) continue;
+            uint32_t e = h ? P.e1 : P.e0;
+            double ca = h ? P.a1c : P.a0c, sa = h ? P.a1s : P.a0s;
+            uint32_t rx = rank1(e), ry = rank2(e), par = par_of(e);
+            for (int l = sp.lane; l < nl; l += sp.L) {
+                double2 b = gang[l];
+                double c = ca * b.x - sa * b.y;
+                double s = sa * b.x + ca * b.y;
+                double mu = par ? s : -s;
+                rotate(tile[l * C + rx], tile[l * C + ry], (R)c, (R)mu);
+            }
+        }
+    } else if (P.type == OP_AAAA || P.type == OP_BBBB) {
+        const R c = (R)P.cs;
+        for (int h = 0; h < 2; ++h) {
+            int k = h ? sp.k1 : sp.k0;
+            if (k < 0) continue;
+            uint32_t e = h ? P.e1 : P.e0;
+            uint32_t r01 = rank1(e), r10 = rank2(e);
+            const R mu = (R)(par_of(e) ? P.sn : -P.sn);
+            for (int l = sp.lane; l < nl; l += sp.L) rotate(tile[l * C + r01], tile[l * C + r10], c, mu);
+        }
+    } else {  // OP_ABBA on a COL tile: alpha CA(s,p) x beta column pairs (q,r)
+        if (P.aux != cur_qr) {
+            cur_qr = P.aux;
+            const int q = (int)(P.aux & 0xff), r = (int)(P.aux >> 8);
+            if (threadIdx.x < 64) {
+                const int t = threadIdx.x;
+                bool is01 = false;
+                int partner = -1, parb = 0;
+                if (t < nl) {
+                    uint32_t s = T.line_str[t];
+                    if (((s >> r) & 1u) && !((s >> q) & 1u)) {
+                        uint32_t ps = s ^ (1u << q) ^ (1u << r);
+                        for (int l = 0; l < nl; ++l)
+                            if (T.line_str[l] == ps) partner = l;
+                        is01 = partner >= 0;
+                        parb = popcnt(s & between_mask(q, r)) & 1;
+                    }
+                }
+                uint64_t mask = __ballot(is01);
+                if (is01) {
+                    int slot = __popcll(mask & ((1ull << t) - 1ull));
+                    T.pair_list[slot] = t | (partner << 8) | (parb << 16);
+                }
+                if (t == 0) *T.pair_count = __popcll(mask);
+            }
+            lds_barrier();
+        }
+        const int ncp = *T.pair_count;
+        const R c = (R)P.cs;
+        for (int h = 0; h < 2; ++h) {
+            int k = h ? sp.k1 : sp.k0;
+            if (k < 0) continue;
+            uint32_t e = h ? P.e1 : P.e0;
+            uint32_t rs_ = rank1(e), rp_ = rank2(e), para = par_of(e);
+            for (int x = sp.lane; x < ncp; x += sp.L) {
+                int pl = T.pair_list[x];
+                int l01 = pl & 0xff, l10 = (pl >> 8) & 0xff, parb = (pl >> 16) & 1;
+                const R mu = (R)(((para + parb) & 1) ? P.sn : -P.sn);
+                rotate(tile[l01 * C + rs_], tile[l10 * C + rp_], c, mu);
+            }
+        }
+    }
 }
 
 template <typename R>
@@ -86,12 +270,11 @@ __global__ void __launch_bounds__(kThreads)
 pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
     using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t *tab = reinterpret_cast<uint32_t *>(smem);                 // binomials
-    uint32_t *line_rank = tab + kBinomWords;                            // [kMaxLines]
+    uint32_t *line_rank = reinterpret_cast<uint32_t *>(smem);            // [kMaxLines]
     uint32_t *line_str = line_rank + kMaxLines;                         // [kMaxLines]
-    int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines] slot01 | slot10<<8 | par<<16
-    int *pair_count = pair_list + kMaxLines;                            // [1] (+pad)
-    double *red = reinterpret_cast<double *>(pair_count + 4);           // [kThreads/64 * 2]
+    int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines]
+    int *pair_count = pair_list + kMaxLines;                            // [4]
+    double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kThreads/64]
     V *tile = reinterpret_cast<V *>(smem + 4096);
 
     const int tid = threadIdx.x;
@@ -99,14 +282,20 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     const int nl = (int)it.count;
     const uint32_t C = A.C;
     const bool col = pass.kind == PASS_COL;
+    double2 *gang = reinterpret_cast<double2 *>(smem + 4096 + (size_t)pass.max_lines * C * sizeof(V));
 
-    for (int i = tid; i < kBinomWords; i += kThreads) tab[i] = A.binom[i];
     if (tid < nl) {
         uint32_t r = col ? A.col_list[it.begin + tid] : it.begin + tid;
         line_rank[tid] = r;
         line_str[tid] = A.strings[r];
     }
     __syncthreads();
+    // staged group angles: the other-string part of each group's angle per line
+    for (int i = tid; i < pass.group_count * nl; i += kThreads) {
+        int g = i / nl, l = i - g * nl;
+        const double *gb = A.grp_b + 2ull * (pass.group_first + g) * C + 2ull * line_rank[l];
+        gang[i] = make_double2(gb[0], gb[1]);
+    }
 
     // ---- load the tile ---------------------------------------------------
     const bool from_prev = pass.flags & PF_FROM_PREV;
@@ -130,94 +319,28 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
             tile[l * C + b] = v;
         }
     }
+    __syncthreads();
 
-    // ---- apply the ops in order -------------------------------------------
-    for (int o = pass.op_begin; o < pass.op_end; ++o) {
-        const OpDev op = A.ops[o];
-        __syncthreads();
-        if (op.type == OP_DIAG) {
-            for (uint32_t idx = tid; idx < total; idx += kThreads) {
-                uint32_t l = idx / C, a = idx - l * C;
-                uint32_t b = line_rank[l];
-                uint32_t sa = A.strings[a];
-                double th = A.theta_a[a] + A.theta_b[b];
-                while (sa) {
-                    int p = __builtin_ctz(sa);
-                    th += A.cross[(uint64_t)p * C + b];
-                    sa &= sa - 1;
-                }
-                double sn, cs;
-                sincos(th, &sn, &cs);
-                V v = tile[idx], w;
-                R c = (R)cs, s = (R)(-sn);
-                w.x = c * v.x - s * v.y;
-                w.y = c * v.y + s * v.x;
-                tile[idx] = w;
-            }
-        } else if (op.type == OP_GA || op.type == OP_GB) {
-            const GroupDev g = A.groups[op.index];
-            const double *ga = A.grp_a + 2ull * op.index * A.ca_pairs;
-            const double *gb = A.grp_b + 2ull * op.index * C;
-            const int np = (int)A.ca_pairs;
-            const int lp = lanes_per_pair(np, nl);
-            for (int k = tid / lp; k < np; k += kThreads / lp) {
-                uint32_t rx, ry, par;
-                ca_pair((uint32_t)k, g.x, g.y, A.n, A.e, tab, &rx, &ry, &par);
-                double ca = ga[2 * k], sa = ga[2 * k + 1];
-                for (int l = tid % lp; l < nl; l += lp) {
-                    uint32_t other = line_rank[l];
-                    double cb = gb[2 * other], sb = gb[2 * other + 1];
-                    double c = ca * cb - sa * sb;
-                    double s = sa * cb + ca * sb;
-                    double mu = par ? s : -s;
-                    rotate(tile[l * C + rx], tile[l * C + ry], (R)c, (R)mu);
-                }
-            }
-        } else if (op.type == OP_AAAA || op.type == OP_BBBB) {
-            const TermDev T = A.terms[op.index];
-            const int np = (int)A.ccaa_pairs;
-            const int lp = lanes_per_pair(np, nl);
-            const R c = (R)T.cs;
-            for (int k = tid / lp; k < np; k += kThreads / lp) {
-                uint32_t r01, r10, par;
-                ccaa_pair((uint32_t)k, T.p, T.q, T.r, T.s, A.n, A.e, tab, &r01, &r10, &par);
-                const R mu = (R)(par ? T.sn : -T.sn);
-                for (int l = tid % lp; l < nl; l += lp) rotate(tile[l * C + r01], tile[l * C + r10], c, mu);
-            }
-        } else {  // OP_ABBA on a COL tile: alpha CA(p,s) x beta column pairs (q,r)
-            const TermDev T = A.terms[op.index];
-            if (tid == 0) *pair_count = 0;
-            __syncthreads();
-            if (tid < nl) {
-                uint32_t s = line_str[tid];
-                if (((s >> T.r) & 1u) && !((s >> T.q) & 1u)) {
-                    uint32_t ps = s ^ (1u << T.q) ^ (1u << T.r);
-                    int partner = -1;
-                    for (int l = 0; l < nl; ++l)
-                        if (line_str[l] == ps) partner = l;
-                    if (partner >= 0) {
-                        int parb = popcnt(s & between_mask(T.q, T.r)) & 1;
-                        int slot = atomicAdd(pair_count, 1);
-                        pair_list[slot] = tid | (partner << 8) | (parb << 16);
-                    }
-                }
-            }
-            __syncthreads();
-            const int ncp = *pair_count;
-            const int np = (int)A.ca_pairs;
-            const int lp = lanes_per_pair(np, ncp);
-            const R c = (R)T.cs;
-            for (int k = tid / lp; k < np; k += kThreads / lp) {
-                uint32_t rs_, rp_, para;
-                ca_pair((uint32_t)k, T.s, T.p, A.n, A.e, tab, &rs_, &rp_, &para);
-                for (int x = tid % lp; x < ncp; x += lp) {
-                    int pl = pair_list[x];
-                    int l01 = pl & 0xff, l10 = (pl >> 8) & 0xff, parb = (pl >> 16) & 1;
-                    const R mu = (R)(((para + parb) & 1) ? T.sn : -T.sn);
-                    rotate(tile[l01 * C + rs_], tile[l10 * C + rp_], c, mu);
-                }
-            }
-        }
+    // ---- apply the ops in order, fetching kPrefetch ops ahead --------------
+    Tile<R> T;
+    T.v = tile; T.gang = gang; T.line_rank = line_rank; T.line_str = line_str;
+    T.pair_list = pair_list; T.pair_count = pair_count; T.nl = nl; T.C = C;
+    uint32_t cur_qr = 0xffffffffu;
+    const int o0 = pass.op_begin, o1 = pass.op_end;
+    OpPre p0, p1, p2, p3;
+    fetch_op(A, o0, o1, p0);
+    fetch_op(A, o0 + 1, o1, p1);
+    fetch_op(A, o0 + 2, o1, p2);
+    fetch_op(A, o0 + 3, o1, p3);
+    for (int o = o0; o < o1; o += 4) {
+        apply_op<R>(A, T, p0, cur_qr);
+        fetch_op(A, o + 4, o1, p0);
+        apply_op<R>(A, T, p1, cur_qr);
+        fetch_op(A, o + 5, o1, p1);
+        apply_op<R>(A, T, p2, cur_qr);
+        fetch_op(A, o + 6, o1, p2);
+        apply_op<R>(A, T, p3, cur_qr);
+        fetch_op(A, o + 7, o1, p3);
     }
     __syncthreads();
 
@@ -527,20 +650,26 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     to_device(&ops, p.ops, s);
     to_device(&items, p.items, s);
     to_device(&col_list, p.col_list, s);
+    to_device(&ca_table, p.ca_table, s);
+    to_device(&ccaa_table, p.ccaa_table, s);
     hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
-    int maxlines = 0;
-    for (const auto &ps : p.passes) maxlines = std::max(maxlines, ps.max_lines);
-    if (maxlines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
-    lds_bytes = 4096 + (size_t)maxlines * p.C * p.elem_bytes;
+    lds_bytes = 0;
+    for (const auto &ps : p.passes) {
+        if (ps.max_lines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
+        size_t b = 4096 + (size_t)ps.max_lines * p.C * p.elem_bytes + (size_t)ps.group_count * ps.max_lines * 16;
+        lds_bytes = std::max(lds_bytes, b);
+    }
     hip_check(hipStreamSynchronize(s), "plan upload sync");
 }
 
 void DevicePlan::release() {
-    void *ptrs[] = {binom, strings, theta_a, theta_b, cross, groups, grp_a, grp_b, terms, ops, items, col_list, partials};
+    void *ptrs[] = {binom, strings, theta_a, theta_b, cross, groups, grp_a, grp_b, terms, ops, items, col_list,
+                    partials, ca_table, ccaa_table};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     binom = nullptr; strings = nullptr; theta_a = theta_b = cross = nullptr; groups = nullptr;
     grp_a = grp_b = nullptr; terms = nullptr; ops = nullptr; items = nullptr; col_list = nullptr; partials = nullptr;
+    ca_table = ccaa_table = nullptr;
 }
 
 template <typename R>
@@ -553,6 +682,7 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norm
     A.theta_a = dp.theta_a; A.theta_b = dp.theta_b; A.cross = dp.cross;
     A.groups = dp.groups; A.grp_a = dp.grp_a; A.grp_b = dp.grp_b;
     A.terms = dp.terms; A.ops = dp.ops; A.items = dp.items; A.col_list = dp.col_list;
+    A.ca_table = dp.ca_table; A.ccaa_table = dp.ccaa_table;
     A.partials = dp.partials; A.scale = scale;
     A.n = p.n; A.e = p.e; A.C = p.C; A.ca_pairs = p.ca_pairs; A.ccaa_pairs = p.ccaa_pairs;
     A.renorm = renorm;

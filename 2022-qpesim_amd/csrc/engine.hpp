@@ -25,8 +25,8 @@ struct DevicePlan {
     OpDev *ops = nullptr;
     ItemDev *items = nullptr;
     uint32_t *col_list = nullptr;
+    uint32_t *ca_table = nullptr, *ccaa_table = nullptr;
     double *partials = nullptr;   // [max_items] norm partial sums
-    int block_threads = 512;
     size_t lds_bytes = 0;         // dynamic LDS per pass launch
 
     void upload(const Plan &p, int dtype, hipStream_t s);

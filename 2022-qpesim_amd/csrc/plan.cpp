@@ -55,6 +55,19 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     P.strings.resize(C);
     for (uint32_t i = 0; i < C; ++i) P.strings[i] = unrank_of(i, e, n, B.tab);
 
+    if (C > kRankMask) throw std::runtime_error("C(num_orbitals, num_occupied) exceeds the packed-rank range");
+    // single-excitation pair tables for every ordered orbital pair (x, y)
+    P.ca_table.assign((size_t)n * n * P.ca_pairs, 0u);
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y < n; ++y) {
+            if (x == y) continue;
+            for (uint32_t k = 0; k < P.ca_pairs; ++k) {
+                uint32_t rx, ry, par;
+                ca_pair(k, x, y, n, e, B.tab, &rx, &ry, &par);
+                P.ca_table[((size_t)x * n + y) * P.ca_pairs + k] = pack_pair(rx, ry, par);
+            }
+        }
+
     auto spin_of = [n](int g) { return g < n ? 0 : 1; };
     auto local = [n](int g) { return g < n ? g : g - n; };
 
@@ -162,6 +175,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     // ---- pqrs terms (TEOT:519-560) --------------------------------------
     struct Rot { int cls; int index; int p, q, r, s; };
     std::vector<Rot> rots;
+    std::vector<uint32_t> cc_offset;   // per term: offset in ccaa_table
     for (const auto &T : seg[kPQRS]) {
         int p = T.idx[0], q = T.idx[1], r = T.idx[2], s = T.idx[3];
         int cls;
@@ -175,16 +189,56 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         d.cs = std::cos(a);
         d.sn = std::sin(a);
         rots.push_back({cls, (int)P.terms.size(), d.p, d.q, d.r, d.s});
+        if (cls != OP_ABBA) {
+            uint32_t base = (uint32_t)P.ccaa_table.size();
+            cc_offset.push_back(base);
+            for (uint32_t k = 0; k < P.ccaa_pairs; ++k) {
+                uint32_t r01, r10, par;
+                ccaa_pair(k, d.p, d.q, d.r, d.s, n, e, B.tab, &r01, &r10, &par);
+                P.ccaa_table.push_back(pack_pair(r01, r10, par));
+            }
+        } else {
+            cc_offset.push_back(0);
+        }
         P.terms.push_back(d);
         P.pair_updates += (cls == OP_ABBA) ? (uint64_t)npa * npa : (uint64_t)P.ccaa_pairs * C;
     }
 
     // ---- passes ----------------------------------------------------------
     const uint64_t line_bytes = (uint64_t)C * elem_bytes;
-    const int W = (int)std::min<uint64_t>(tile_bytes / line_bytes, 64);
+    // lines per workgroup for a pass that stages `ngroups` group angle rows
+    auto lines_for = [&](int ngroups) {
+        uint64_t per_line = line_bytes + (uint64_t)ngroups * 16u;
+        return (int)std::min<uint64_t>(tile_bytes / per_line, 64);
+    };
+    const int W = lines_for(0);
     if (W < 1) throw std::runtime_error("a single column does not fit the tile budget");
+    const uint32_t nca = P.ca_pairs;
+    auto ca_off = [&](int x, int y) { return (uint32_t)(((size_t)x * n + y) * nca); };
+    auto op_group = [&](int type, int g, int slot) {
+        OpDev o;
+        o.type = type;
+        o.index = g;
+        o.tab = ca_off(P.groups[g].x, P.groups[g].y);
+        o.aux = (uint32_t)slot;
+        return o;
+    };
+    auto op_term = [&](const Rot &R) {
+        OpDev o;
+        o.type = R.cls;
+        o.index = R.index;
+        if (R.cls == OP_ABBA) {
+            o.tab = ca_off(R.s, R.p);        // alpha pair: rank(f|s), rank(f|p)
+            o.aux = (uint32_t)R.q | ((uint32_t)R.r << 8);
+        } else {
+            o.tab = cc_offset[R.index];
+            o.aux = 0;
+        }
+        return o;
+    };
 
-    auto add_pass = [&](int kind, int flags, int op_begin, std::vector<ItemDev> items, int max_lines) {
+    auto add_pass = [&](int kind, int flags, int op_begin, std::vector<ItemDev> items, int max_lines,
+                        int gfirst, int gcount) {
         PassDev pd;
         pd.kind = kind;
         pd.flags = flags;
@@ -193,7 +247,9 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         pd.item_begin = (int)P.items.size();
         pd.item_count = (int)items.size();
         pd.max_lines = max_lines;
-        pd.pad = 0;
+        pd.group_first = gfirst;
+        pd.group_count = gcount;
+        pd.pad0 = pd.pad1 = pd.pad2 = 0;
         for (const auto &it : items) {
             P.items.push_back(it);
             P.tile_bytes_moved += 2ull * it.count * line_bytes;
@@ -201,37 +257,47 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         P.max_items = std::max(P.max_items, pd.item_count);
         P.passes.push_back(pd);
     };
-    auto chunk_columns = [&]() {
+    auto chunk_columns = [&](int w) {
         std::vector<ItemDev> items;
-        for (uint32_t c0 = 0; c0 < C; c0 += (uint32_t)W) {
+        for (uint32_t c0 = 0; c0 < C; c0 += (uint32_t)w) {
             ItemDev it;
             it.begin = (uint32_t)P.col_list.size();
-            it.count = std::min<uint32_t>((uint32_t)W, C - c0);
+            it.count = std::min<uint32_t>((uint32_t)w, C - c0);
             for (uint32_t c = 0; c < it.count; ++c) P.col_list.push_back(c0 + c);
             items.push_back(it);
         }
         return items;
     };
-    auto chunk_rows = [&]() {
+    auto chunk_rows = [&](int w) {
         std::vector<ItemDev> items;
-        for (uint32_t r0 = 0; r0 < C; r0 += (uint32_t)W) items.push_back({r0, std::min<uint32_t>((uint32_t)W, C - r0)});
+        for (uint32_t r0 = 0; r0 < C; r0 += (uint32_t)w) items.push_back({r0, std::min<uint32_t>((uint32_t)w, C - r0)});
         return items;
     };
 
+    // groups are stored alpha first, then beta (pq map order), so each spin is a contiguous range
+    int ga_first = -1, ga_count = 0, gb_first = -1, gb_count = 0;
+    for (size_t g = 0; g < P.groups.size(); ++g) {
+        if (P.groups[g].spin == 0) { if (ga_first < 0) ga_first = (int)g; ++ga_count; }
+        else { if (gb_first < 0) gb_first = (int)g; ++gb_count; }
+    }
     // pass 0: copy block j-1 -> j, diagonal phase, alpha groups (COL)
     {
+        int w = lines_for(ga_count);
+        if (w < 1) throw std::runtime_error("tile budget too small for the alpha excitation groups");
         int ob = (int)P.ops.size();
-        P.ops.push_back({OP_DIAG, 0});
-        for (size_t g = 0; g < P.groups.size(); ++g)
-            if (P.groups[g].spin == 0) P.ops.push_back({OP_GA, (int)g});
-        add_pass(PASS_COL, PF_FROM_PREV, ob, chunk_columns(), W);
+        OpDev d;
+        d.type = OP_DIAG; d.index = 0; d.tab = 0; d.aux = 0;
+        P.ops.push_back(d);
+        for (int g = 0; g < ga_count; ++g) P.ops.push_back(op_group(OP_GA, ga_first + g, g));
+        add_pass(PASS_COL, PF_FROM_PREV, ob, chunk_columns(w), w, ga_count ? ga_first : 0, ga_count);
     }
     // pass 1: beta groups (ROW)
-    {
+    if (gb_count) {
+        int w = lines_for(gb_count);
+        if (w < 1) throw std::runtime_error("tile budget too small for the beta excitation groups");
         int ob = (int)P.ops.size();
-        for (size_t g = 0; g < P.groups.size(); ++g)
-            if (P.groups[g].spin == 1) P.ops.push_back({OP_GB, (int)g});
-        if ((int)P.ops.size() > ob) add_pass(PASS_ROW, 0, ob, chunk_rows(), W);
+        for (int g = 0; g < gb_count; ++g) P.ops.push_back(op_group(OP_GB, gb_first + g, g));
+        add_pass(PASS_ROW, 0, ob, chunk_rows(w), w, gb_first, gb_count);
     }
     // alpha-leading double excitations, p by p: AAAA_p then ABBA_p (map order)
     auto max_orbit = [&](int tsize, int k0, int k1) {
@@ -256,7 +322,6 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         while (ai < abba.size()) {
             // window: consecutive terms sharing q, over a run of r values
             int q = abba[ai].q;
-            std::vector<int> rs;
             size_t aj = ai;
             uint32_t tmask = 1u << q;
             while (aj < abba.size() && abba[aj].q == q) {
@@ -280,8 +345,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             k_range(ts, pending_aaaa, &k0, &k1);
             int ob = (int)P.ops.size();
             if (pending_aaaa)
-                for (const auto &R : aaaa) P.ops.push_back({OP_AAAA, R.index});
-            for (size_t x = ai; x < aj; ++x) P.ops.push_back({OP_ABBA, abba[x].index});
+                for (const auto &R : aaaa) P.ops.push_back(op_term(R));
+            for (size_t x = ai; x < aj; ++x) P.ops.push_back(op_term(abba[x]));
             // orbits: outside string y (e-k electrons on n-ts positions) x all k-patterns of T
             std::vector<ItemDev> items;
             ItemDev cur{(uint32_t)P.col_list.size(), 0};
@@ -301,28 +366,28 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
                 }
             }
             if (cur.count) items.push_back(cur);
-            add_pass(PASS_COL, 0, ob, items, W);
+            add_pass(PASS_COL, 0, ob, items, W, 0, 0);
             pending_aaaa = false;
             ai = aj;
         }
         if (pending_aaaa) {
             int ob = (int)P.ops.size();
-            for (const auto &R : aaaa) P.ops.push_back({OP_AAAA, R.index});
-            add_pass(PASS_COL, 0, ob, chunk_columns(), W);
+            for (const auto &R : aaaa) P.ops.push_back(op_term(R));
+            add_pass(PASS_COL, 0, ob, chunk_columns(W), W, 0, 0);
         }
     }
     // pure-beta double excitations (ROW)
     {
         int ob = (int)P.ops.size();
-        for (; ri < rots.size(); ++ri) P.ops.push_back({OP_BBBB, rots[ri].index});
-        if ((int)P.ops.size() > ob) add_pass(PASS_ROW, 0, ob, chunk_rows(), W);
+        for (; ri < rots.size(); ++ri) P.ops.push_back(op_term(rots[ri]));
+        if ((int)P.ops.size() > ob) add_pass(PASS_ROW, 0, ob, chunk_rows(W), W, 0, 0);
     }
     // the block norm is summed by the last pass; it must see every amplitude
     {
         const PassDev &last = P.passes.back();
         uint64_t covered = 0;
         for (int i = 0; i < last.item_count; ++i) covered += P.items[last.item_begin + i].count;
-        if (covered != C) add_pass(PASS_COL, 0, (int)P.ops.size(), chunk_columns(), W);
+        if (covered != C) add_pass(PASS_COL, 0, (int)P.ops.size(), chunk_columns(W), W, 0, 0);
     }
     P.passes.back().flags |= PF_EMIT_NORM;
     return P;

@@ -50,7 +50,17 @@ enum PassFlags : int32_t {
 struct OpDev {
     int32_t type;
     int32_t index;   // group index (GA/GB) or term index (pqrs)
+    uint32_t tab;    // offset of this op's pair entries in the CA or CCAA table
+    uint32_t aux;    // GA/GB: staged-angle slot; ABBA: q | r << 8 (beta orbitals)
 };
+
+// A pair entry packs two ranks and the Jordan-Wigner parity into 32 bits:
+// bits 0-14 rank of the first string, 15-29 rank of the second, 31 parity.
+constexpr uint32_t kRankBits = 15;
+constexpr uint32_t kRankMask = (1u << kRankBits) - 1u;
+inline uint32_t pack_pair(uint32_t r1, uint32_t r2, uint32_t par) {
+    return r1 | (r2 << kRankBits) | (par << 31);
+}
 
 struct GroupDev {
     int32_t x, y;    // orbitals of the excitation inside its spin string (x > y)
@@ -69,7 +79,9 @@ struct PassDev {
     int32_t op_begin, op_end;
     int32_t item_begin, item_count;
     int32_t max_lines;    // columns (COL) or rows (ROW) per workgroup
-    int32_t pad;
+    int32_t group_first;  // first excitation group whose angle table is staged in LDS
+    int32_t group_count;  // number of staged groups (0: none)
+    int32_t pad0, pad1, pad2;
 };
 
 struct ItemDev {
@@ -95,6 +107,9 @@ struct Plan {
     std::vector<PassDev> passes;
     std::vector<ItemDev> items;
     std::vector<uint32_t> col_list;     // column ranks of COL items
+    std::vector<uint32_t> ca_table;     // [(x*n+y)*ca_pairs + k]: rank(f|x), rank(f|y), parity
+    std::vector<uint32_t> ccaa_table;   // [cc*ccaa_pairs + k]: rank(f|r|s), rank(f|p|q), parity
+    size_t lds_fixed = 4096;            // LDS bytes before the tile
 
     // work accounting per U
     uint64_t pair_updates = 0;   // determinant-pair rotations of the reference's terms
