@@ -25,7 +25,8 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-constexpr int kThreads = 1024;
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / 64;
 constexpr int kMaxLines = 64;
 constexpr int kBinomWords = kBinomRows * kBinomCols;
 
@@ -83,185 +84,195 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Work split of one op: a thread owns up to two determinant pairs (k0, k1) and
-// the lines lane, lane + L, ...  (L threads share one pair).
-struct Split {
-    int L, lane, k0, k1;
-};
-
-__device__ inline Split split_for(int np) {
-    Split sp;
-    const int t = threadIdx.x;
-    if (np <= 0) {
-        sp.L = 1; sp.lane = 0; sp.k0 = -1; sp.k1 = -1;
-    } else if (np <= kThreads) {
-        sp.L = kThreads / np;
-        sp.k0 = t < sp.L * np ? t % np : -1;
-        sp.lane = t < sp.L * np ? t / np : 0;
-        sp.k1 = -1;
-    } else {
-        sp.L = 1; sp.lane = 0;
-        sp.k0 = t;
-        sp.k1 = t + kThreads < np ? t + kThreads : -1;
-    }
-    return sp;
-}
-
-// Everything one op needs that does not depend on amplitudes; fetched
-// kPrefetch ops ahead so that its global-memory latency overlaps earlier ops.
-struct OpPre {
-    int type, index;
-    uint32_t aux;
-    int np;
-    uint32_t e0, e1;
-    double a0c, a0s, a1c, a1s;
-    double cs, sn;
-};
-
-__device__ inline void fetch_op(const PassArgs &A, int o, int o_end, OpPre &P) {
-    P.type = -1;
-    if (o >= o_end) return;
-    const OpDev od = A.ops[o];
-    P.type = od.type;
-    P.index = od.index;
-    P.aux = od.aux;
-    const uint32_t *tab = nullptr;
-    P.np = 0;
-    P.cs = 1.0; P.sn = 0.0;
-    if (od.type == OP_AAAA || od.type == OP_BBBB) {
-        P.np = (int)A.ccaa_pairs;
-        tab = A.ccaa_table + od.tab;
-    } else if (od.type != OP_DIAG) {
-        P.np = (int)A.ca_pairs;
-        tab = A.ca_table + od.tab;
-    }
-    if (od.type >= OP_AAAA) {
-        P.cs = A.terms[od.index].cs;
-        P.sn = A.terms[od.index].sn;
-    }
-    const Split sp = split_for(P.np);
-    P.e0 = sp.k0 >= 0 ? tab[sp.k0] : 0u;
-    P.e1 = sp.k1 >= 0 ? tab[sp.k1] : 0u;
-    P.a0c = 1.0; P.a0s = 0.0; P.a1c = 1.0; P.a1s = 0.0;
-    if (od.type == OP_GA || od.type == OP_GB) {
-        const double *ga = A.grp_a + 2ull * od.index * A.ca_pairs;
-        if (sp.k0 >= 0) { P.a0c = ga[2 * sp.k0]; P.a0s = ga[2 * sp.k0 + 1]; }
-        if (sp.k1 >= 0) { P.a1c = ga[2 * sp.k1]; P.a1s = ga[2 * sp.k1 + 1]; }
-    }
-}
-
 __device__ inline uint32_t rank1(uint32_t e) { return e & kRankMask; }
 __device__ inline uint32_t rank2(uint32_t e) { return (e >> kRankBits) & kRankMask; }
 __device__ inline uint32_t par_of(uint32_t e) { return e >> 31; }
 
+// Work of one op inside a wavefront: lane ln owns determinant pairs
+// k = ln + 64*c (c < kIter); the wavefront owns whole lines (or, for mixed
+// alpha-beta runs, whole column pairs), so line-local ops need no block
+// barrier: one wavefront's LDS accesses are processed in issue order.
+constexpr int kIter = 8;
+
+struct OpStage {
+    int type, index;
+    uint32_t tab, aux;
+    double cs, sn;
+    int np;
+};
+
+struct Entries {
+    uint32_t e[kIter];
+};
+
+__device__ inline OpStage load_desc(const PassArgs &A, int o, int o_end) {
+    OpStage d;
+    d.type = -1;
+    d.index = 0; d.tab = 0; d.aux = 0; d.cs = 1.0; d.sn = 0.0; d.np = 0;
+    if (o >= o_end) return d;
+    const OpDev od = A.ops[o];
+    d.type = od.type; d.index = od.index; d.tab = od.tab; d.aux = od.aux; d.cs = od.cs; d.sn = od.sn;
+    d.np = (od.type == OP_DIAG) ? 0 : (od.type == OP_AAAA || od.type == OP_BBBB) ? (int)A.ccaa_pairs : (int)A.ca_pairs;
+    return d;
+}
+
+__device__ inline void load_entries(const PassArgs &A, const OpStage &d, int ln, Entries &E) {
+    const uint32_t *tab = (d.type == OP_AAAA || d.type == OP_BBBB) ? A.ccaa_table : A.ca_table;
+#pragma unroll
+    for (int c = 0; c < kIter; ++c) {
+        int k = ln + 64 * c;
+        E.e[c] = (d.type > OP_DIAG && k < d.np) ? tab[d.tab + k] : 0u;
+    }
+}
+
 template <typename R>
-struct Tile {
-    typename Vec<R>::type *v;      // [nl][C]
-    double2 *gang;                 // staged group angles [group slot][nl]
+struct TileCtx {
+    typename Vec<R>::type *v;    // [nl][C]
+    double2 *gang;               // staged group angles [slot][nl]
     const uint32_t *line_rank, *line_str;
     int *pair_list, *pair_count;
-    int nl;
+    int nl, wave, lane;
     uint32_t C;
 };
 
+// rotate pair entries of one op on one line (line-local ops)
 template <typename R>
-__device__ inline void apply_op(const PassArgs &A, Tile<R> &T, const OpPre &P, uint32_t &cur_qr) {
+__device__ inline void rotate_line(typename Vec<R>::type *line, const Entries &E, int np, int ln, R c, double sn) {
     using V = typename Vec<R>::type;
-    if (P.type < 0) return;
-    lds_barrier();
-    const int nl = T.nl;
-    const uint32_t C = T.C;
-    V *tile = T.v;
-    if (P.type == OP_DIAG) {
-        const uint32_t total = (uint32_t)nl * C;
-        for (uint32_t idx = threadIdx.x; idx < total; idx += kThreads) {
-            uint32_t l = idx / C, a = idx - l * C;
-            uint32_t b = T.line_rank[l];
-            uint32_t sa = A.strings[a];
-            double th = A.theta_a[a] + A.theta_b[b];
-            while (sa) {
-                int p = __builtin_ctz(sa);
-                th += A.cross[(uint64_t)p * C + b];
-                sa &= sa - 1;
-            }
-            double sn, cs;
-            sincos(th, &sn, &cs);
-            V v = tile[idx], w;
-            R c = (R)cs, s = (R)(-sn);
-            w.x = c * v.x - s * v.y;
-            w.y = c * v.y + s * v.x;
-            tile[idx] = w;
-        }
-        return;
+    V u[kIter], w[kIter];
+    uint32_t r1[kIter], r2[kIter];
+#pragma unroll
+    for (int i = 0; i < kIter; ++i) {
+        r1[i] = rank1(E.e[i]);
+        r2[i] = rank2(E.e[i]);
+        if (ln + 64 * i < np) { u[i] = line[r1[i]]; w[i] = line[r2[i]]; }
     }
-    const Split sp = split_for(P.np);
-    if (P.type == OP_GA || P.type == OP_GB) {
-        const double2 *gang = T.gang + (size_t)P.aux * nl;
-        for (int h = 0; h < 2; ++h) {
-            int k = h ? sp.k1 : sp.k0;
-            if (k < 0) continue;
-            uint32_t e = h ? P.e1 : P.e0;
-            double ca = h ? P.a1c : P.a0c, sa = h ? P.a1s : P.a0s;
-            uint32_t rx = rank1(e), ry = rank2(e), par = par_of(e);
-            for (int l = sp.lane; l < nl; l += sp.L) {
-                double2 b = gang[l];
-                double c = ca * b.x - sa * b.y;
-                double s = sa * b.x + ca * b.y;
-                double mu = par ? s : -s;
-                rotate(tile[l * C + rx], tile[l * C + ry], (R)c, (R)mu);
-            }
+#pragma unroll
+    for (int i = 0; i < kIter; ++i) {
+        if (ln + 64 * i < np) {
+            const R mu = (R)(par_of(E.e[i]) ? sn : -sn);
+            rotate(u[i], w[i], c, mu);
+            line[r1[i]] = u[i];
+            line[r2[i]] = w[i];
         }
-    } else if (P.type == OP_AAAA || P.type == OP_BBBB) {
-        const R c = (R)P.cs;
-        for (int h = 0; h < 2; ++h) {
-            int k = h ? sp.k1 : sp.k0;
-            if (k < 0) continue;
-            uint32_t e = h ? P.e1 : P.e0;
-            uint32_t r01 = rank1(e), r10 = rank2(e);
-            const R mu = (R)(par_of(e) ? P.sn : -P.sn);
-            for (int l = sp.lane; l < nl; l += sp.L) rotate(tile[l * C + r01], tile[l * C + r10], c, mu);
-        }
-    } else {  // OP_ABBA on a COL tile: alpha CA(s,p) x beta column pairs (q,r)
-        if (P.aux != cur_qr) {
-            cur_qr = P.aux;
-            const int q = (int)(P.aux & 0xff), r = (int)(P.aux >> 8);
-            if (threadIdx.x < 64) {
-                const int t = threadIdx.x;
-                bool is01 = false;
-                int partner = -1, parb = 0;
-                if (t < nl) {
-                    uint32_t s = T.line_str[t];
-                    if (((s >> r) & 1u) && !((s >> q) & 1u)) {
-                        uint32_t ps = s ^ (1u << q) ^ (1u << r);
-                        for (int l = 0; l < nl; ++l)
-                            if (T.line_str[l] == ps) partner = l;
-                        is01 = partner >= 0;
-                        parb = popcnt(s & between_mask(q, r)) & 1;
-                    }
+    }
+}
+
+template <typename R>
+__device__ inline void apply_line_op(const PassArgs &A, TileCtx<R> &T, const OpStage &d, const Entries &E) {
+    using V = typename Vec<R>::type;
+    const uint32_t C = T.C;
+    const int ln = T.lane;
+    if (d.type == OP_DIAG) {
+        for (int l = T.wave; l < T.nl; l += kWaves) {
+            V *line = T.v + (size_t)l * C;
+            const uint32_t b = T.line_rank[l];
+            for (uint32_t a = ln; a < C; a += 64) {
+                uint32_t sa = A.strings[a];
+                double th = A.theta_a[a] + A.theta_b[b];
+                while (sa) {
+                    int p = __builtin_ctz(sa);
+                    th += A.cross[(uint64_t)p * C + b];
+                    sa &= sa - 1;
                 }
-                uint64_t mask = __ballot(is01);
-                if (is01) {
-                    int slot = __popcll(mask & ((1ull << t) - 1ull));
-                    T.pair_list[slot] = t | (partner << 8) | (parb << 16);
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                V v = line[a], w;
+                R c = (R)cs, s = (R)(-sn);
+                w.x = c * v.x - s * v.y;
+                w.y = c * v.y + s * v.x;
+                line[a] = w;
+            }
+        }
+    } else if (d.type == OP_GA || d.type == OP_GB) {
+        // angle = pair part (per k, FP64 table) + other-string part (per line, staged in LDS)
+        const double *ga = A.grp_a + 2ull * d.index * A.ca_pairs;
+        double ca[kIter], sa[kIter];
+#pragma unroll
+        for (int i = 0; i < kIter; ++i) {
+            int k = ln + 64 * i;
+            ca[i] = k < d.np ? ga[2 * k] : 1.0;
+            sa[i] = k < d.np ? ga[2 * k + 1] : 0.0;
+        }
+        const double2 *gang = T.gang + (size_t)d.aux * T.nl;
+        for (int l = T.wave; l < T.nl; l += kWaves) {
+            V *line = T.v + (size_t)l * C;
+            const double2 b = gang[l];
+            V u[kIter], w[kIter];
+#pragma unroll
+            for (int i = 0; i < kIter; ++i)
+                if (ln + 64 * i < d.np) { u[i] = line[rank1(E.e[i])]; w[i] = line[rank2(E.e[i])]; }
+#pragma unroll
+            for (int i = 0; i < kIter; ++i) {
+                if (ln + 64 * i < d.np) {
+                    double c = ca[i] * b.x - sa[i] * b.y;
+                    double s = sa[i] * b.x + ca[i] * b.y;
+                    double mu = par_of(E.e[i]) ? s : -s;
+                    rotate(u[i], w[i], (R)c, (R)mu);
+                    line[rank1(E.e[i])] = u[i];
+                    line[rank2(E.e[i])] = w[i];
                 }
-                if (t == 0) *T.pair_count = __popcll(mask);
-            }
-            lds_barrier();
-        }
-        const int ncp = *T.pair_count;
-        const R c = (R)P.cs;
-        for (int h = 0; h < 2; ++h) {
-            int k = h ? sp.k1 : sp.k0;
-            if (k < 0) continue;
-            uint32_t e = h ? P.e1 : P.e0;
-            uint32_t rs_ = rank1(e), rp_ = rank2(e), para = par_of(e);
-            for (int x = sp.lane; x < ncp; x += sp.L) {
-                int pl = T.pair_list[x];
-                int l01 = pl & 0xff, l10 = (pl >> 8) & 0xff, parb = (pl >> 16) & 1;
-                const R mu = (R)(((para + parb) & 1) ? P.sn : -P.sn);
-                rotate(tile[l01 * C + rs_], tile[l10 * C + rp_], c, mu);
             }
         }
+    } else {   // AAAA / BBBB
+        const R c = (R)d.cs;
+        for (int l = T.wave; l < T.nl; l += kWaves)
+            rotate_line<R>(T.v + (size_t)l * C, E, d.np, ln, c, d.sn);
+    }
+}
+
+// mixed alpha-beta op on the column pairs (q, r) this wavefront owns
+template <typename R>
+__device__ inline void apply_abba(TileCtx<R> &T, const OpStage &d, const Entries &E, int ncp) {
+    using V = typename Vec<R>::type;
+    const uint32_t C = T.C;
+    const int ln = T.lane;
+    const R c = (R)d.cs;
+    for (int x = T.wave; x < ncp; x += kWaves) {
+        const int pl = T.pair_list[x];
+        V *l01 = T.v + (size_t)(pl & 0xff) * C;
+        V *l10 = T.v + (size_t)((pl >> 8) & 0xff) * C;
+        const int parb = (pl >> 16) & 1;
+        V u[kIter], w[kIter];
+#pragma unroll
+        for (int i = 0; i < kIter; ++i)
+            if (ln + 64 * i < d.np) { u[i] = l01[rank1(E.e[i])]; w[i] = l10[rank2(E.e[i])]; }
+#pragma unroll
+        for (int i = 0; i < kIter; ++i) {
+            if (ln + 64 * i < d.np) {
+                const R mu = (R)(((par_of(E.e[i]) + parb) & 1) ? d.sn : -d.sn);
+                rotate(u[i], w[i], c, mu);
+                l01[rank1(E.e[i])] = u[i];
+                l10[rank2(E.e[i])] = w[i];
+            }
+        }
+    }
+}
+
+// build the (q, r) column-pair list of the tile in LDS (first wavefront)
+template <typename R>
+__device__ inline void build_pairs(TileCtx<R> &T, uint32_t aux) {
+    const int q = (int)(aux & 0xff), r = (int)(aux >> 8);
+    if (threadIdx.x < 64) {
+        const int t = threadIdx.x;
+        bool is01 = false;
+        int partner = -1, parb = 0;
+        if (t < T.nl) {
+            uint32_t s = T.line_str[t];
+            if (((s >> r) & 1u) && !((s >> q) & 1u)) {
+                uint32_t ps = s ^ (1u << q) ^ (1u << r);
+                for (int l = 0; l < T.nl; ++l)
+                    if (T.line_str[l] == ps) partner = l;
+                is01 = partner >= 0;
+                parb = popcnt(s & between_mask(q, r)) & 1;
+            }
+        }
+        uint64_t mask = __ballot(is01);
+        if (is01) {
+            int slot = __popcll(mask & ((1ull << t) - 1ull));
+            T.pair_list[slot] = t | (partner << 8) | (parb << 16);
+        }
+        if (t == 0) *T.pair_count = __popcll(mask);
     }
 }
 
@@ -274,7 +285,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     uint32_t *line_str = line_rank + kMaxLines;                         // [kMaxLines]
     int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines]
     int *pair_count = pair_list + kMaxLines;                            // [4]
-    double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kThreads/64]
+    double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kWaves]
     V *tile = reinterpret_cast<V *>(smem + 4096);
 
     const int tid = threadIdx.x;
@@ -290,7 +301,6 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
         line_str[tid] = A.strings[r];
     }
     __syncthreads();
-    // staged group angles: the other-string part of each group's angle per line
     for (int i = tid; i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
         const double *gb = A.grp_b + 2ull * (pass.group_first + g) * C + 2ull * line_rank[l];
@@ -321,26 +331,36 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     }
     __syncthreads();
 
-    // ---- apply the ops in order, fetching kPrefetch ops ahead --------------
-    Tile<R> T;
+    // ---- ops: descriptor two ahead, pair entries one ahead, apply ----------
+    TileCtx<R> T;
     T.v = tile; T.gang = gang; T.line_rank = line_rank; T.line_str = line_str;
     T.pair_list = pair_list; T.pair_count = pair_count; T.nl = nl; T.C = C;
-    uint32_t cur_qr = 0xffffffffu;
+    T.wave = tid >> 6;
+    T.lane = tid & 63;
     const int o0 = pass.op_begin, o1 = pass.op_end;
-    OpPre p0, p1, p2, p3;
-    fetch_op(A, o0, o1, p0);
-    fetch_op(A, o0 + 1, o1, p1);
-    fetch_op(A, o0 + 2, o1, p2);
-    fetch_op(A, o0 + 3, o1, p3);
-    for (int o = o0; o < o1; o += 4) {
-        apply_op<R>(A, T, p0, cur_qr);
-        fetch_op(A, o + 4, o1, p0);
-        apply_op<R>(A, T, p1, cur_qr);
-        fetch_op(A, o + 5, o1, p1);
-        apply_op<R>(A, T, p2, cur_qr);
-        fetch_op(A, o + 6, o1, p2);
-        apply_op<R>(A, T, p3, cur_qr);
-        fetch_op(A, o + 7, o1, p3);
+    uint32_t mode = 0xfffffffeu;   // 0xffffffff: line-local; else the (q, r) of the current mixed run
+    int ncp = 0;
+    OpStage cur_d = load_desc(A, o0, o1), next_d = load_desc(A, o0 + 1, o1);
+    Entries cur_e, next_e;
+    load_entries(A, cur_d, T.lane, cur_e);
+    for (int o = o0; o < o1; ++o) {
+        OpStage after_d = load_desc(A, o + 2, o1);
+        load_entries(A, next_d, T.lane, next_e);
+        const uint32_t want = (cur_d.type == OP_ABBA) ? cur_d.aux : 0xffffffffu;
+        if (want != mode) {
+            lds_barrier();
+            if (cur_d.type == OP_ABBA) {
+                build_pairs<R>(T, cur_d.aux);
+                lds_barrier();
+                ncp = *pair_count;
+            }
+            mode = want;
+        }
+        if (cur_d.type == OP_ABBA) apply_abba<R>(T, cur_d, cur_e, ncp);
+        else apply_line_op<R>(A, T, cur_d, cur_e);
+        cur_d = next_d;
+        next_d = after_d;
+        cur_e = next_e;
     }
     __syncthreads();
 
@@ -363,7 +383,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
         __syncthreads();
         if (tid == 0) {
             KSum tot;
-            for (int w = 0; w < kThreads / 64; ++w) {
+            for (int w = 0; w < kWaves; ++w) {
                 tot.add(red[2 * w]);
                 tot.add(red[2 * w + 1]);
             }

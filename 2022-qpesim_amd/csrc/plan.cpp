@@ -56,6 +56,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     for (uint32_t i = 0; i < C; ++i) P.strings[i] = unrank_of(i, e, n, B.tab);
 
     if (C > kRankMask) throw std::runtime_error("C(num_orbitals, num_occupied) exceeds the packed-rank range");
+    if (P.ca_pairs > kMaxPairs || P.ccaa_pairs > kMaxPairs)
+        throw std::runtime_error("more determinant pairs per term than the tile engine unrolls");
     // single-excitation pair tables for every ordered orbital pair (x, y)
     P.ca_table.assign((size_t)n * n * P.ca_pairs, 0u);
     for (int x = 0; x < n; ++x)
@@ -221,6 +223,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         o.index = g;
         o.tab = ca_off(P.groups[g].x, P.groups[g].y);
         o.aux = (uint32_t)slot;
+        o.cs = 1.0;
+        o.sn = 0.0;
         return o;
     };
     auto op_term = [&](const Rot &R) {
@@ -234,6 +238,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             o.tab = cc_offset[R.index];
             o.aux = 0;
         }
+        o.cs = P.terms[R.index].cs;
+        o.sn = P.terms[R.index].sn;
         return o;
     };
 
@@ -286,7 +292,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         if (w < 1) throw std::runtime_error("tile budget too small for the alpha excitation groups");
         int ob = (int)P.ops.size();
         OpDev d;
-        d.type = OP_DIAG; d.index = 0; d.tab = 0; d.aux = 0;
+        d.type = OP_DIAG; d.index = 0; d.tab = 0; d.aux = 0; d.cs = 1.0; d.sn = 0.0;
         P.ops.push_back(d);
         for (int g = 0; g < ga_count; ++g) P.ops.push_back(op_group(OP_GA, ga_first + g, g));
         add_pass(PASS_COL, PF_FROM_PREV, ob, chunk_columns(w), w, ga_count ? ga_first : 0, ga_count);

@@ -52,10 +52,14 @@ struct OpDev {
     int32_t index;   // group index (GA/GB) or term index (pqrs)
     uint32_t tab;    // offset of this op's pair entries in the CA or CCAA table
     uint32_t aux;    // GA/GB: staged-angle slot; ABBA: q | r << 8 (beta orbitals)
+    double cs, sn;   // pqrs: cos(c t), sin(c t)
 };
 
 // A pair entry packs two ranks and the Jordan-Wigner parity into 32 bits:
 // bits 0-14 rank of the first string, 15-29 rank of the second, 31 parity.
+// pairs of one term a wavefront handles per op: 64 lanes x 8 unrolled iterations
+constexpr uint32_t kMaxPairs = 512;
+
 constexpr uint32_t kRankBits = 15;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1u;
 inline uint32_t pack_pair(uint32_t r1, uint32_t r2, uint32_t par) {

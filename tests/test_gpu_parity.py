@@ -155,8 +155,9 @@ def test_fp32_precision_form(n, e, m, seed):
     ra, ry, rP = oracle_run(n, e, m, seed, hf=0.9)
     assert np.max(np.abs(P - rP)) <= PK_TOL_F32
     assert rel_l2(amps, ra) < 1e-4
-    # every block is renormalised to block 0's norm, i.e. to |psi0|^2 after rounding to FP32
-    assert abs(P.sum() - norms[0] * (1 << m)) < 1e-12
+    # every block is renormalised to block 0's norm, i.e. to |psi0|^2 after rounding to FP32;
+    # reading the amplitudes back folds the factors into the FP32 register (relative 1e-8 rounding)
+    assert abs(P.sum() - norms[0] * (1 << m)) < 1e-8
     assert abs(P.sum() - 1.0) < 1e-6
 
 
