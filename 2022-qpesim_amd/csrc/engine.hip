@@ -88,191 +88,73 @@ __device__ inline uint32_t rank1(uint32_t e) { return e & kRankMask; }
 __device__ inline uint32_t rank2(uint32_t e) { return (e >> kRankBits) & kRankMask; }
 __device__ inline uint32_t par_of(uint32_t e) { return e >> 31; }
 
-// Work of one op inside a wavefront: lane ln owns determinant pairs
-// k = ln + 64*c (c < kIter); the wavefront owns whole lines (or, for mixed
-// alpha-beta runs, whole column pairs), so line-local ops need no block
-// barrier: one wavefront's LDS accesses are processed in issue order.
-constexpr int kIter = 8;
+// Work split of one op over the workgroup: thread t owns the determinant pairs
+// k = kk + i*np... of its class, i.e. k0 = t mod np (and k0 + kThreads,
+// k0 + 2*kThreads when np > kThreads) and the lines lane, lane + L, ...
+// with L = max(1, kThreads / np) threads per pair.  Computed once per kernel
+// for the two pair classes (single excitations CA, double excitations CCAA).
+constexpr int kMaxEnt = 3;   // np <= 3 * kThreads
+
+struct Split {
+    int k0, lane, L, nent;   // nent: pairs this thread owns (0..kMaxEnt)
+};
+
+__device__ inline Split make_split(int np) {
+    Split sp;
+    const int t = threadIdx.x;
+    if (np <= 0) {
+        sp.k0 = 0; sp.lane = 0; sp.L = 1; sp.nent = 0;
+    } else if (np <= kThreads) {
+        sp.L = kThreads / np;
+        sp.k0 = t % np;
+        sp.lane = t / np;
+        sp.nent = sp.lane < sp.L ? 1 : 0;
+    } else {
+        sp.L = 1; sp.lane = 0; sp.k0 = t;
+        sp.nent = (np - t + kThreads - 1) / kThreads;
+        if (sp.nent > kMaxEnt) sp.nent = kMaxEnt;
+    }
+    return sp;
+}
 
 struct OpStage {
     int type, index;
     uint32_t tab, aux;
     double cs, sn;
-    int np;
-};
-
-struct Entries {
-    uint32_t e[kIter];
 };
 
 __device__ inline OpStage load_desc(const PassArgs &A, int o, int o_end) {
     OpStage d;
-    d.type = -1;
-    d.index = 0; d.tab = 0; d.aux = 0; d.cs = 1.0; d.sn = 0.0; d.np = 0;
-    if (o >= o_end) return d;
+    if (o >= o_end) {
+        d.type = -1; d.index = 0; d.tab = 0; d.aux = 0; d.cs = 1.0; d.sn = 0.0;
+        return d;
+    }
     const OpDev od = A.ops[o];
     d.type = od.type; d.index = od.index; d.tab = od.tab; d.aux = od.aux; d.cs = od.cs; d.sn = od.sn;
-    d.np = (od.type == OP_DIAG) ? 0 : (od.type == OP_AAAA || od.type == OP_BBBB) ? (int)A.ccaa_pairs : (int)A.ca_pairs;
     return d;
 }
 
-__device__ inline void load_entries(const PassArgs &A, const OpStage &d, int ln, Entries &E) {
-    const uint32_t *tab = (d.type == OP_AAAA || d.type == OP_BBBB) ? A.ccaa_table : A.ca_table;
-#pragma unroll
-    for (int c = 0; c < kIter; ++c) {
-        int k = ln + 64 * c;
-        E.e[c] = (d.type > OP_DIAG && k < d.np) ? tab[d.tab + k] : 0u;
-    }
-}
-
-template <typename R>
-struct TileCtx {
-    typename Vec<R>::type *v;    // [nl][C]
-    double2 *gang;               // staged group angles [slot][nl]
-    const uint32_t *line_rank, *line_str;
-    int *pair_list, *pair_count;
-    int nl, wave, lane;
-    uint32_t C;
+struct Ent {
+    uint32_t e[kMaxEnt];
+    double ac[kMaxEnt], as[kMaxEnt];   // GA/GB: cos, sin of the pair part of the angle
 };
 
-// rotate pair entries of one op on one line (line-local ops)
-template <typename R>
-__device__ inline void rotate_line(typename Vec<R>::type *line, const Entries &E, int np, int ln, R c, double sn) {
-    using V = typename Vec<R>::type;
-    V u[kIter], w[kIter];
-    uint32_t r1[kIter], r2[kIter];
+__device__ inline void load_ent(const PassArgs &A, const OpStage &d, const Split &ca, const Split &cc, Ent &E) {
+    if (d.type <= OP_DIAG) return;
+    const bool dbl = d.type == OP_AAAA || d.type == OP_BBBB;
+    const Split &sp = dbl ? cc : ca;
+    const uint32_t *tab = (dbl ? A.ccaa_table : A.ca_table) + d.tab;
 #pragma unroll
-    for (int i = 0; i < kIter; ++i) {
-        r1[i] = rank1(E.e[i]);
-        r2[i] = rank2(E.e[i]);
-        if (ln + 64 * i < np) { u[i] = line[r1[i]]; w[i] = line[r2[i]]; }
-    }
-#pragma unroll
-    for (int i = 0; i < kIter; ++i) {
-        if (ln + 64 * i < np) {
-            const R mu = (R)(par_of(E.e[i]) ? sn : -sn);
-            rotate(u[i], w[i], c, mu);
-            line[r1[i]] = u[i];
-            line[r2[i]] = w[i];
-        }
-    }
-}
-
-template <typename R>
-__device__ inline void apply_line_op(const PassArgs &A, TileCtx<R> &T, const OpStage &d, const Entries &E) {
-    using V = typename Vec<R>::type;
-    const uint32_t C = T.C;
-    const int ln = T.lane;
-    if (d.type == OP_DIAG) {
-        for (int l = T.wave; l < T.nl; l += kWaves) {
-            V *line = T.v + (size_t)l * C;
-            const uint32_t b = T.line_rank[l];
-            for (uint32_t a = ln; a < C; a += 64) {
-                uint32_t sa = A.strings[a];
-                double th = A.theta_a[a] + A.theta_b[b];
-                while (sa) {
-                    int p = __builtin_ctz(sa);
-                    th += A.cross[(uint64_t)p * C + b];
-                    sa &= sa - 1;
-                }
-                double sn, cs;
-                sincos(th, &sn, &cs);
-                V v = line[a], w;
-                R c = (R)cs, s = (R)(-sn);
-                w.x = c * v.x - s * v.y;
-                w.y = c * v.y + s * v.x;
-                line[a] = w;
-            }
-        }
-    } else if (d.type == OP_GA || d.type == OP_GB) {
-        // angle = pair part (per k, FP64 table) + other-string part (per line, staged in LDS)
+    for (int i = 0; i < kMaxEnt; ++i)
+        if (i < sp.nent) E.e[i] = tab[sp.k0 + i * kThreads];
+    if (d.type == OP_GA || d.type == OP_GB) {
         const double *ga = A.grp_a + 2ull * d.index * A.ca_pairs;
-        double ca[kIter], sa[kIter];
 #pragma unroll
-        for (int i = 0; i < kIter; ++i) {
-            int k = ln + 64 * i;
-            ca[i] = k < d.np ? ga[2 * k] : 1.0;
-            sa[i] = k < d.np ? ga[2 * k + 1] : 0.0;
-        }
-        const double2 *gang = T.gang + (size_t)d.aux * T.nl;
-        for (int l = T.wave; l < T.nl; l += kWaves) {
-            V *line = T.v + (size_t)l * C;
-            const double2 b = gang[l];
-            V u[kIter], w[kIter];
-#pragma unroll
-            for (int i = 0; i < kIter; ++i)
-                if (ln + 64 * i < d.np) { u[i] = line[rank1(E.e[i])]; w[i] = line[rank2(E.e[i])]; }
-#pragma unroll
-            for (int i = 0; i < kIter; ++i) {
-                if (ln + 64 * i < d.np) {
-                    double c = ca[i] * b.x - sa[i] * b.y;
-                    double s = sa[i] * b.x + ca[i] * b.y;
-                    double mu = par_of(E.e[i]) ? s : -s;
-                    rotate(u[i], w[i], (R)c, (R)mu);
-                    line[rank1(E.e[i])] = u[i];
-                    line[rank2(E.e[i])] = w[i];
-                }
+        for (int i = 0; i < kMaxEnt; ++i)
+            if (i < sp.nent) {
+                E.ac[i] = ga[2 * (sp.k0 + i * kThreads)];
+                E.as[i] = ga[2 * (sp.k0 + i * kThreads) + 1];
             }
-        }
-    } else {   // AAAA / BBBB
-        const R c = (R)d.cs;
-        for (int l = T.wave; l < T.nl; l += kWaves)
-            rotate_line<R>(T.v + (size_t)l * C, E, d.np, ln, c, d.sn);
-    }
-}
-
-// mixed alpha-beta op on the column pairs (q, r) this wavefront owns
-template <typename R>
-__device__ inline void apply_abba(TileCtx<R> &T, const OpStage &d, const Entries &E, int ncp) {
-    using V = typename Vec<R>::type;
-    const uint32_t C = T.C;
-    const int ln = T.lane;
-    const R c = (R)d.cs;
-    for (int x = T.wave; x < ncp; x += kWaves) {
-        const int pl = T.pair_list[x];
-        V *l01 = T.v + (size_t)(pl & 0xff) * C;
-        V *l10 = T.v + (size_t)((pl >> 8) & 0xff) * C;
-        const int parb = (pl >> 16) & 1;
-        V u[kIter], w[kIter];
-#pragma unroll
-        for (int i = 0; i < kIter; ++i)
-            if (ln + 64 * i < d.np) { u[i] = l01[rank1(E.e[i])]; w[i] = l10[rank2(E.e[i])]; }
-#pragma unroll
-        for (int i = 0; i < kIter; ++i) {
-            if (ln + 64 * i < d.np) {
-                const R mu = (R)(((par_of(E.e[i]) + parb) & 1) ? d.sn : -d.sn);
-                rotate(u[i], w[i], c, mu);
-                l01[rank1(E.e[i])] = u[i];
-                l10[rank2(E.e[i])] = w[i];
-            }
-        }
-    }
-}
-
-// build the (q, r) column-pair list of the tile in LDS (first wavefront)
-template <typename R>
-__device__ inline void build_pairs(TileCtx<R> &T, uint32_t aux) {
-    const int q = (int)(aux & 0xff), r = (int)(aux >> 8);
-    if (threadIdx.x < 64) {
-        const int t = threadIdx.x;
-        bool is01 = false;
-        int partner = -1, parb = 0;
-        if (t < T.nl) {
-            uint32_t s = T.line_str[t];
-            if (((s >> r) & 1u) && !((s >> q) & 1u)) {
-                uint32_t ps = s ^ (1u << q) ^ (1u << r);
-                for (int l = 0; l < T.nl; ++l)
-                    if (T.line_str[l] == ps) partner = l;
-                is01 = partner >= 0;
-                parb = popcnt(s & between_mask(q, r)) & 1;
-            }
-        }
-        uint64_t mask = __ballot(is01);
-        if (is01) {
-            int slot = __popcll(mask & ((1ull << t) - 1ull));
-            T.pair_list[slot] = t | (partner << 8) | (parb << 16);
-        }
-        if (t == 0) *T.pair_count = __popcll(mask);
     }
 }
 
@@ -314,11 +196,14 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     if (from_prev && A.renorm) sc = (R)A.scale[j - 1];
     const uint32_t total = (uint32_t)nl * C;
     if (col) {
-        for (uint32_t idx = tid; idx < total; idx += kThreads) {
-            uint32_t l = idx / C, a = idx - l * C;
-            V v = src[(uint64_t)line_rank[l] * C + a];
-            v.x *= sc; v.y *= sc;
-            tile[idx] = v;
+        for (int l = 0; l < nl; ++l) {
+            const V *s_line = src + (uint64_t)line_rank[l] * C;
+            V *t_line = tile + (size_t)l * C;
+            for (uint32_t a = tid; a < C; a += kThreads) {
+                V v = s_line[a];
+                v.x *= sc; v.y *= sc;
+                t_line[a] = v;
+            }
         }
     } else {
         const uint32_t r0 = it.begin;
@@ -331,36 +216,109 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     }
     __syncthreads();
 
-    // ---- ops: descriptor two ahead, pair entries one ahead, apply ----------
-    TileCtx<R> T;
-    T.v = tile; T.gang = gang; T.line_rank = line_rank; T.line_str = line_str;
-    T.pair_list = pair_list; T.pair_count = pair_count; T.nl = nl; T.C = C;
-    T.wave = tid >> 6;
-    T.lane = tid & 63;
+    // ---- ops in order: block barrier per op, descriptors two ahead, pair
+    // entries one ahead (global loads stay in flight across LDS barriers) ----
+    const Split sca = make_split((int)A.ca_pairs), scc = make_split((int)A.ccaa_pairs);
     const int o0 = pass.op_begin, o1 = pass.op_end;
-    uint32_t mode = 0xfffffffeu;   // 0xffffffff: line-local; else the (q, r) of the current mixed run
+    uint32_t cur_qr = 0xffffffffu;
     int ncp = 0;
-    OpStage cur_d = load_desc(A, o0, o1), next_d = load_desc(A, o0 + 1, o1);
-    Entries cur_e, next_e;
-    load_entries(A, cur_d, T.lane, cur_e);
+    OpStage d0 = load_desc(A, o0, o1), d1 = load_desc(A, o0 + 1, o1);
+    Ent e0, e1;
+    load_ent(A, d0, sca, scc, e0);
     for (int o = o0; o < o1; ++o) {
-        OpStage after_d = load_desc(A, o + 2, o1);
-        load_entries(A, next_d, T.lane, next_e);
-        const uint32_t want = (cur_d.type == OP_ABBA) ? cur_d.aux : 0xffffffffu;
-        if (want != mode) {
-            lds_barrier();
-            if (cur_d.type == OP_ABBA) {
-                build_pairs<R>(T, cur_d.aux);
+        const OpStage d2 = load_desc(A, o + 2, o1);
+        load_ent(A, d1, sca, scc, e1);
+        lds_barrier();
+        if (d0.type == OP_DIAG) {
+            for (uint32_t idx = tid; idx < total; idx += kThreads) {
+                uint32_t l = idx / C, a = idx - l * C;
+                uint32_t b = line_rank[l];
+                uint32_t sa = A.strings[a];
+                double th = A.theta_a[a] + A.theta_b[b];
+                while (sa) {
+                    int p = __builtin_ctz(sa);
+                    th += A.cross[(uint64_t)p * C + b];
+                    sa &= sa - 1;
+                }
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                V v = tile[idx], w;
+                R c = (R)cs, s = (R)(-sn);
+                w.x = c * v.x - s * v.y;
+                w.y = c * v.y + s * v.x;
+                tile[idx] = w;
+            }
+        } else if (d0.type == OP_ABBA) {
+            if (d0.aux != cur_qr) {
+                cur_qr = d0.aux;
+                const int q = (int)(cur_qr & 0xff), r = (int)(cur_qr >> 8);
+                if (tid < 64) {
+                    bool is01 = false;
+                    int partner = -1, parb = 0;
+                    if (tid < nl) {
+                        uint32_t s = line_str[tid];
+                        if (((s >> r) & 1u) && !((s >> q) & 1u)) {
+                            uint32_t ps = s ^ (1u << q) ^ (1u << r);
+                            for (int l = 0; l < nl; ++l)
+                                if (line_str[l] == ps) partner = l;
+                            is01 = partner >= 0;
+                            parb = popcnt(s & between_mask(q, r)) & 1;
+                        }
+                    }
+                    uint64_t mask = __ballot(is01);
+                    if (is01) pair_list[__popcll(mask & ((1ull << tid) - 1ull))] = tid | (partner << 8) | (parb << 16);
+                    if (tid == 0) *pair_count = __popcll(mask);
+                }
                 lds_barrier();
                 ncp = *pair_count;
             }
-            mode = want;
+            const R c = (R)d0.cs;
+#pragma unroll
+            for (int i = 0; i < kMaxEnt; ++i) {
+                if (i >= sca.nent) break;
+                const uint32_t e = e0.e[i];
+                const uint32_t rs_ = rank1(e), rp_ = rank2(e), para = par_of(e);
+                for (int x = sca.lane; x < ncp; x += sca.L) {
+                    const int pl = pair_list[x];
+                    V *l01 = tile + (size_t)(pl & 0xff) * C;
+                    V *l10 = tile + (size_t)((pl >> 8) & 0xff) * C;
+                    const R mu = (R)(((para ^ (uint32_t)(pl >> 16)) & 1u) ? d0.sn : -d0.sn);
+                    rotate(l01[rs_], l10[rp_], c, mu);
+                }
+            }
+        } else if (d0.type == OP_GA || d0.type == OP_GB) {
+            const double2 *ga = gang + (size_t)d0.aux * nl;
+#pragma unroll
+            for (int i = 0; i < kMaxEnt; ++i) {
+                if (i >= sca.nent) break;
+                const uint32_t e = e0.e[i];
+                const uint32_t rx = rank1(e), ry = rank2(e), par = par_of(e);
+                const double ca = e0.ac[i], sa = e0.as[i];
+                for (int l = sca.lane; l < nl; l += sca.L) {
+                    const double2 b = ga[l];
+                    const double c = ca * b.x - sa * b.y;
+                    const double s = sa * b.x + ca * b.y;
+                    V *line = tile + (size_t)l * C;
+                    rotate(line[rx], line[ry], (R)c, (R)(par ? s : -s));
+                }
+            }
+        } else if (d0.type >= 0) {   // AAAA / BBBB
+            const R c = (R)d0.cs;
+#pragma unroll
+            for (int i = 0; i < kMaxEnt; ++i) {
+                if (i >= scc.nent) break;
+                const uint32_t e = e0.e[i];
+                const uint32_t r01 = rank1(e), r10 = rank2(e);
+                const R mu = (R)(par_of(e) ? d0.sn : -d0.sn);
+                for (int l = scc.lane; l < nl; l += scc.L) {
+                    V *line = tile + (size_t)l * C;
+                    rotate(line[r01], line[r10], c, mu);
+                }
+            }
         }
-        if (cur_d.type == OP_ABBA) apply_abba<R>(T, cur_d, cur_e, ncp);
-        else apply_line_op<R>(A, T, cur_d, cur_e);
-        cur_d = next_d;
-        next_d = after_d;
-        cur_e = next_e;
+        d0 = d1;
+        d1 = d2;
+        e0 = e1;
     }
     __syncthreads();
 
@@ -393,9 +351,10 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
 
     // ---- store the tile -----------------------------------------------------
     if (col) {
-        for (uint32_t idx = tid; idx < total; idx += kThreads) {
-            uint32_t l = idx / C, a = idx - l * C;
-            cur[(uint64_t)line_rank[l] * C + a] = tile[idx];
+        for (int l = 0; l < nl; ++l) {
+            V *d_line = cur + (uint64_t)line_rank[l] * C;
+            const V *t_line = tile + (size_t)l * C;
+            for (uint32_t a = tid; a < C; a += kThreads) d_line[a] = t_line[a];
         }
     } else {
         const uint32_t r0 = it.begin;

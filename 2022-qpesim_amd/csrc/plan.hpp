@@ -57,8 +57,8 @@ struct OpDev {
 
 // A pair entry packs two ranks and the Jordan-Wigner parity into 32 bits:
 // bits 0-14 rank of the first string, 15-29 rank of the second, 31 parity.
-// pairs of one term a wavefront handles per op: 64 lanes x 8 unrolled iterations
-constexpr uint32_t kMaxPairs = 512;
+// pairs of one term a workgroup handles per op: 512 threads x 3 pairs each
+constexpr uint32_t kMaxPairs = 1536;
 
 constexpr uint32_t kRankBits = 15;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1u;
