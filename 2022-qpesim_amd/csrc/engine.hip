@@ -117,44 +117,147 @@ __device__ inline Split make_split(int np) {
     return sp;
 }
 
-struct OpStage {
-    int type, index;
-    uint32_t tab, aux;
-    double cs, sn;
+// Op pipeline.  Ops run in batches of kBatch.  Descriptors of batch b+1 sit
+// in an LDS ring (written one batch ahead, so reading them needs no global
+// load); each thread's pair entries for batch b+1 are loaded into registers
+// while batch b runs and handed over at the batch boundary, so their latency
+// overlaps a whole batch.  No scalar loads happen inside the loop: scalar and
+// LDS traffic share one wait counter.
+constexpr int kBatch = 8;
+constexpr int kWaitVmcnt0 = 0x0F70;   // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15) (gfx9 encoding)
+constexpr int kDescWords = sizeof(OpDev) / 4;   // 8
+
+struct BatchEnt {
+    uint32_t e[kBatch][kMaxEnt];
 };
 
-__device__ inline OpStage load_desc(const PassArgs &A, int o, int o_end) {
-    OpStage d;
-    if (o >= o_end) {
-        d.type = -1; d.index = 0; d.tab = 0; d.aux = 0; d.cs = 1.0; d.sn = 0.0;
-        return d;
+__device__ inline const OpDev &desc_at(const OpDev *ring, int slot, int i) { return ring[slot * kBatch + i]; }
+
+// stage the descriptors of batch b (ops o0 + b*kBatch ...) into ring slot b % 2
+__device__ inline void stage_descs(const PassArgs &A, OpDev *ring, int o0, int o1, int b) {
+    const int first = o0 + b * kBatch;
+    const int w = threadIdx.x;
+    if (w < kBatch * kDescWords) {
+        const int i = w / kDescWords, word = w - i * kDescWords;
+        uint32_t v = 0;
+        if (first + i < o1) v = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
+        reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[w] = v;
     }
-    const OpDev od = A.ops[o];
-    d.type = od.type; d.index = od.index; d.tab = od.tab; d.aux = od.aux; d.cs = od.cs; d.sn = od.sn;
-    return d;
 }
 
-struct Ent {
-    uint32_t e[kMaxEnt];
-    double ac[kMaxEnt], as[kMaxEnt];   // GA/GB: cos, sin of the pair part of the angle
-};
+__device__ inline void load_batch(const PassArgs &A, const OpDev *ring, int o0, int o1, int b, const Split &ca,
+                                  const Split &cc, BatchEnt &E) {
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+        const int o = o0 + b * kBatch + i;
+        const OpDev &d = desc_at(ring, b & 1, i);
+        const int type = o < o1 ? d.type : -1;
+        const bool dbl = type == OP_AAAA || type == OP_BBBB;
+        const Split &sp = dbl ? cc : ca;
+        const uint32_t *tab = (dbl ? A.ccaa_table : A.ca_table) + d.tab;
+#pragma unroll
+        for (int x = 0; x < kMaxEnt; ++x)
+            E.e[i][x] = (type > OP_DIAG && x < sp.nent) ? tab[sp.k0 + x * kThreads] : 0u;
+    }
+}
 
-__device__ inline void load_ent(const PassArgs &A, const OpStage &d, const Split &ca, const Split &cc, Ent &E) {
-    if (d.type <= OP_DIAG) return;
-    const bool dbl = d.type == OP_AAAA || d.type == OP_BBBB;
-    const Split &sp = dbl ? cc : ca;
-    const uint32_t *tab = (dbl ? A.ccaa_table : A.ca_table) + d.tab;
-#pragma unroll
-    for (int i = 0; i < kMaxEnt; ++i)
-        if (i < sp.nent) E.e[i] = tab[sp.k0 + i * kThreads];
-    if (d.type == OP_GA || d.type == OP_GB) {
-        const double *ga = A.grp_a + 2ull * d.index * A.ca_pairs;
-#pragma unroll
-        for (int i = 0; i < kMaxEnt; ++i)
-            if (i < sp.nent) {
-                E.ac[i] = ga[2 * (sp.k0 + i * kThreads)];
-                E.as[i] = ga[2 * (sp.k0 + i * kThreads) + 1];
+template <typename R>
+__device__ inline void run_op(const PassArgs &A, const OpDev &d, const uint32_t *ent, const Split &sca,
+                              const Split &scc, typename Vec<R>::type *tile, const double2 *gang,
+                              const uint32_t *line_rank, const uint32_t *line_str, int *pair_list, int *pair_count,
+                              int nl, uint32_t C, uint32_t &cur_qr, int &ncp) {
+    using V = typename Vec<R>::type;
+    const int tid = threadIdx.x;
+    const int type = d.type;
+    if (type == OP_DIAG) {
+        const uint32_t total = (uint32_t)nl * C;
+        for (uint32_t idx = tid; idx < total; idx += kThreads) {
+            uint32_t l = idx / C, a = idx - l * C;
+            uint32_t b = line_rank[l];
+            uint32_t sa = A.strings[a];
+            double th = A.theta_a[a] + A.theta_b[b];
+            while (sa) {
+                int p = __builtin_ctz(sa);
+                th += A.cross[(uint64_t)p * C + b];
+                sa &= sa - 1;
             }
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            V v = tile[idx], w;
+            R c = (R)cs, s = (R)(-sn);
+            w.x = c * v.x - s * v.y;
+            w.y = c * v.y + s * v.x;
+            tile[idx] = w;
+        }
+    } else if (type == OP_ABBA) {
+        if (d.aux != cur_qr) {
+            cur_qr = d.aux;
+            const int q = (int)(cur_qr & 0xff), r = (int)(cur_qr >> 8);
+            if (tid < 64) {
+                bool is01 = false;
+                int partner = -1, parb = 0;
+                if (tid < nl) {
+                    uint32_t s = line_str[tid];
+                    if (((s >> r) & 1u) && !((s >> q) & 1u)) {
+                        uint32_t ps = s ^ (1u << q) ^ (1u << r);
+                        for (int l = 0; l < nl; ++l)
+                            if (line_str[l] == ps) partner = l;
+                        is01 = partner >= 0;
+                        parb = popcnt(s & between_mask(q, r)) & 1;
+                    }
+                }
+                uint64_t mask = __ballot(is01);
+                if (is01) pair_list[__popcll(mask & ((1ull << tid) - 1ull))] = tid | (partner << 8) | (parb << 16);
+                if (tid == 0) *pair_count = __popcll(mask);
+            }
+            lds_barrier();
+            ncp = *pair_count;
+        }
+        const R c = (R)d.cs;
+#pragma unroll
+        for (int i = 0; i < kMaxEnt; ++i) {
+            if (i >= sca.nent) break;
+            const uint32_t e = ent[i];
+            const uint32_t rs_ = rank1(e), rp_ = rank2(e), para = par_of(e);
+            for (int x = sca.lane; x < ncp; x += sca.L) {
+                const int pl = pair_list[x];
+                V *l01 = tile + (size_t)(pl & 0xff) * C;
+                V *l10 = tile + (size_t)((pl >> 8) & 0xff) * C;
+                const R mu = (R)(((para ^ (uint32_t)(pl >> 16)) & 1u) ? d.sn : -d.sn);
+                rotate(l01[rs_], l10[rp_], c, mu);
+            }
+        }
+    } else if (type == OP_GA || type == OP_GB) {
+        const double2 *ga = gang + (size_t)d.aux * nl;
+        const double *gp = A.grp_a + 2ull * d.index * A.ca_pairs;
+#pragma unroll
+        for (int i = 0; i < kMaxEnt; ++i) {
+            if (i >= sca.nent) break;
+            const uint32_t e = ent[i];
+            const uint32_t rx = rank1(e), ry = rank2(e), par = par_of(e);
+            const int k = sca.k0 + i * kThreads;
+            const double ca = gp[2 * k], sa = gp[2 * k + 1];
+            for (int l = sca.lane; l < nl; l += sca.L) {
+                const double2 b = ga[l];
+                const double c = ca * b.x - sa * b.y;
+                const double s = sa * b.x + ca * b.y;
+                V *line = tile + (size_t)l * C;
+                rotate(line[rx], line[ry], (R)c, (R)(par ? s : -s));
+            }
+        }
+    } else if (type >= 0) {   // AAAA / BBBB
+        const R c = (R)d.cs;
+#pragma unroll
+        for (int i = 0; i < kMaxEnt; ++i) {
+            if (i >= scc.nent) break;
+            const uint32_t e = ent[i];
+            const uint32_t r01 = rank1(e), r10 = rank2(e);
+            const R mu = (R)(par_of(e) ? d.sn : -d.sn);
+            for (int l = scc.lane; l < nl; l += scc.L) {
+                V *line = tile + (size_t)l * C;
+                rotate(line[r01], line[r10], c, mu);
+            }
+        }
     }
 }
 
@@ -168,6 +271,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines]
     int *pair_count = pair_list + kMaxLines;                            // [4]
     double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kWaves]
+    OpDev *ring = reinterpret_cast<OpDev *>(smem + 2048);              // [2][kBatch]
     V *tile = reinterpret_cast<V *>(smem + 4096);
 
     const int tid = threadIdx.x;
@@ -176,18 +280,23 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     const uint32_t C = A.C;
     const bool col = pass.kind == PASS_COL;
     double2 *gang = reinterpret_cast<double2 *>(smem + 4096 + (size_t)pass.max_lines * C * sizeof(V));
+    const int o0 = pass.op_begin, o1 = pass.op_end;
 
     if (tid < nl) {
         uint32_t r = col ? A.col_list[it.begin + tid] : it.begin + tid;
         line_rank[tid] = r;
         line_str[tid] = A.strings[r];
     }
+    stage_descs(A, ring, o0, o1, 0);
     __syncthreads();
     for (int i = tid; i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
         const double *gb = A.grp_b + 2ull * (pass.group_first + g) * C + 2ull * line_rank[l];
         gang[i] = make_double2(gb[0], gb[1]);
     }
+    const Split sca = make_split((int)A.ca_pairs), scc = make_split((int)A.ccaa_pairs);
+    BatchEnt Ecur, Enext;
+    load_batch(A, ring, o0, o1, 0, sca, scc, Ecur);   // overlaps the tile load below
 
     // ---- load the tile ---------------------------------------------------
     const bool from_prev = pass.flags & PF_FROM_PREV;
@@ -214,111 +323,29 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
             tile[l * C + b] = v;
         }
     }
-    __syncthreads();
 
-    // ---- ops in order: block barrier per op, descriptors two ahead, pair
-    // entries one ahead (global loads stay in flight across LDS barriers) ----
-    const Split sca = make_split((int)A.ca_pairs), scc = make_split((int)A.ccaa_pairs);
-    const int o0 = pass.op_begin, o1 = pass.op_end;
+    // ---- ops ------------------------------------------------------------------
     uint32_t cur_qr = 0xffffffffu;
     int ncp = 0;
-    OpStage d0 = load_desc(A, o0, o1), d1 = load_desc(A, o0 + 1, o1);
-    Ent e0, e1;
-    load_ent(A, d0, sca, scc, e0);
-    for (int o = o0; o < o1; ++o) {
-        const OpStage d2 = load_desc(A, o + 2, o1);
-        load_ent(A, d1, sca, scc, e1);
-        lds_barrier();
-        if (d0.type == OP_DIAG) {
-            for (uint32_t idx = tid; idx < total; idx += kThreads) {
-                uint32_t l = idx / C, a = idx - l * C;
-                uint32_t b = line_rank[l];
-                uint32_t sa = A.strings[a];
-                double th = A.theta_a[a] + A.theta_b[b];
-                while (sa) {
-                    int p = __builtin_ctz(sa);
-                    th += A.cross[(uint64_t)p * C + b];
-                    sa &= sa - 1;
-                }
-                double sn, cs;
-                sincos(th, &sn, &cs);
-                V v = tile[idx], w;
-                R c = (R)cs, s = (R)(-sn);
-                w.x = c * v.x - s * v.y;
-                w.y = c * v.y + s * v.x;
-                tile[idx] = w;
-            }
-        } else if (d0.type == OP_ABBA) {
-            if (d0.aux != cur_qr) {
-                cur_qr = d0.aux;
-                const int q = (int)(cur_qr & 0xff), r = (int)(cur_qr >> 8);
-                if (tid < 64) {
-                    bool is01 = false;
-                    int partner = -1, parb = 0;
-                    if (tid < nl) {
-                        uint32_t s = line_str[tid];
-                        if (((s >> r) & 1u) && !((s >> q) & 1u)) {
-                            uint32_t ps = s ^ (1u << q) ^ (1u << r);
-                            for (int l = 0; l < nl; ++l)
-                                if (line_str[l] == ps) partner = l;
-                            is01 = partner >= 0;
-                            parb = popcnt(s & between_mask(q, r)) & 1;
-                        }
-                    }
-                    uint64_t mask = __ballot(is01);
-                    if (is01) pair_list[__popcll(mask & ((1ull << tid) - 1ull))] = tid | (partner << 8) | (parb << 16);
-                    if (tid == 0) *pair_count = __popcll(mask);
-                }
-                lds_barrier();
-                ncp = *pair_count;
-            }
-            const R c = (R)d0.cs;
+    const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
+    for (int b = 0; b < nbatch; ++b) {
+        // this batch's entries (loaded one batch ago) must land before the next
+        // batch's loads are issued: the vector-memory counter retires in order
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        // descriptors of batch b+1 into the other ring slot, then its entries
+        __syncthreads();
+        if (b + 1 < nbatch) stage_descs(A, ring, o0, o1, b + 1);
+        __syncthreads();
+        if (b + 1 < nbatch) load_batch(A, ring, o0, o1, b + 1, sca, scc, Enext);
 #pragma unroll
-            for (int i = 0; i < kMaxEnt; ++i) {
-                if (i >= sca.nent) break;
-                const uint32_t e = e0.e[i];
-                const uint32_t rs_ = rank1(e), rp_ = rank2(e), para = par_of(e);
-                for (int x = sca.lane; x < ncp; x += sca.L) {
-                    const int pl = pair_list[x];
-                    V *l01 = tile + (size_t)(pl & 0xff) * C;
-                    V *l10 = tile + (size_t)((pl >> 8) & 0xff) * C;
-                    const R mu = (R)(((para ^ (uint32_t)(pl >> 16)) & 1u) ? d0.sn : -d0.sn);
-                    rotate(l01[rs_], l10[rp_], c, mu);
-                }
-            }
-        } else if (d0.type == OP_GA || d0.type == OP_GB) {
-            const double2 *ga = gang + (size_t)d0.aux * nl;
-#pragma unroll
-            for (int i = 0; i < kMaxEnt; ++i) {
-                if (i >= sca.nent) break;
-                const uint32_t e = e0.e[i];
-                const uint32_t rx = rank1(e), ry = rank2(e), par = par_of(e);
-                const double ca = e0.ac[i], sa = e0.as[i];
-                for (int l = sca.lane; l < nl; l += sca.L) {
-                    const double2 b = ga[l];
-                    const double c = ca * b.x - sa * b.y;
-                    const double s = sa * b.x + ca * b.y;
-                    V *line = tile + (size_t)l * C;
-                    rotate(line[rx], line[ry], (R)c, (R)(par ? s : -s));
-                }
-            }
-        } else if (d0.type >= 0) {   // AAAA / BBBB
-            const R c = (R)d0.cs;
-#pragma unroll
-            for (int i = 0; i < kMaxEnt; ++i) {
-                if (i >= scc.nent) break;
-                const uint32_t e = e0.e[i];
-                const uint32_t r01 = rank1(e), r10 = rank2(e);
-                const R mu = (R)(par_of(e) ? d0.sn : -d0.sn);
-                for (int l = scc.lane; l < nl; l += scc.L) {
-                    V *line = tile + (size_t)l * C;
-                    rotate(line[r01], line[r10], c, mu);
-                }
-            }
+        for (int i = 0; i < kBatch; ++i) {
+            if (o0 + b * kBatch + i >= o1) break;
+            const OpDev d = desc_at(ring, b & 1, i);
+            lds_barrier();
+            run_op<R>(A, d, Ecur.e[i], sca, scc, tile, gang, line_rank, line_str, pair_list, pair_count, nl, C,
+                      cur_qr, ncp);
         }
-        d0 = d1;
-        d1 = d2;
-        e0 = e1;
+        Ecur = Enext;
     }
     __syncthreads();
 
