@@ -32,7 +32,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--m", type=int, default=None, help="override the ancilla count (development only)")
     ap.add_argument("--no-fp64", action="store_true")
