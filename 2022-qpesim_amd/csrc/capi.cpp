@@ -350,8 +350,9 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
         switch (option) {
         case QPE_OPT_RENORMALIZE: s->renorm = value ? 1 : 0; break;
         case QPE_OPT_ENGINE:
-            require(value == 0, QPE_ERR_ARG, "only the tile engine (0) is available");
+            require(value == 0 || value == 1, QPE_ERR_ARG, "engine must be 0 (fused groups) or 1 (one term per op)");
             s->engine = (int)value;
+            s->plan_version = ~0ull;
             break;
         case QPE_OPT_MAX_TILE_BYTES:
             require(value >= 0, QPE_ERR_ARG, "negative tile budget");
@@ -376,7 +377,7 @@ int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h) {
             if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
             qpe::Plan p;
             try {
-                p = qpe::build_plan(h->h, s->elem, budget);
+                p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0);
             } catch (const std::runtime_error &e) {
                 throw ApiError(QPE_ERR_ARG, e.what());
             }

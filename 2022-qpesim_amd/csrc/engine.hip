@@ -46,14 +46,18 @@ struct KSum {
     __device__ double value() const { return s + c; }
 };
 
-// 2x2 symmetric rotation {c, i*mu; i*mu, c} on the pair (u, v)
+// 2x2 symmetric rotation {c, i*mu; i*mu, c} on the pair (u, v), given as
+// h = 1 - c.  The whole update -h*u + i*mu*v is formed first and added to u in
+// one rounding: for the small angles of a Trotter step h*u is often below half
+// an ulp of u, and u - h*u rounded on its own would drop it every time, a bias
+// that inflates the norm of complex64 states step after step.
 template <typename V, typename R>
-__device__ inline void rotate(V &u, V &v, R c, R mu) {
+__device__ inline void rotate(V &u, V &v, R h, R mu) {
     V nu, nv;
-    nu.x = c * u.x - mu * v.y;
-    nu.y = c * u.y + mu * v.x;
-    nv.x = c * v.x - mu * u.y;
-    nv.y = c * v.y + mu * u.x;
+    nu.x = u.x + (-h * u.x - mu * v.y);
+    nu.y = u.y + (-h * u.y + mu * v.x);
+    nv.x = v.x + (-h * v.x - mu * u.y);
+    nv.y = v.y + (-h * v.y + mu * u.x);
     u = nu;
     v = nv;
 }
@@ -64,11 +68,11 @@ struct PassArgs {
     const double *theta_a, *theta_b, *cross;
     const GroupDev *groups;
     const double *grp_a, *grp_b;
-    const TermDev *terms;
     const OpDev *ops;
     const ItemDev *items;
     const uint32_t *col_list;
-    const uint32_t *ca_table, *ccaa_table;
+    const uint32_t *sched;
+    const double *coef;
     double *partials;
     const double *scale;
     int n, e;
@@ -84,57 +88,33 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__device__ inline uint32_t rank1(uint32_t e) { return e & kRankMask; }
-__device__ inline uint32_t rank2(uint32_t e) { return (e >> kRankBits) & kRankMask; }
-__device__ inline uint32_t par_of(uint32_t e) { return e >> 31; }
 
-// Work split of one op over the workgroup: thread t owns the determinant pairs
-// k = kk + i*np... of its class, i.e. k0 = t mod np (and k0 + kThreads,
-// k0 + 2*kThreads when np > kThreads) and the lines lane, lane + L, ...
-// with L = max(1, kThreads / np) threads per pair.  Computed once per kernel
-// for the two pair classes (single excitations CA, double excitations CCAA).
-constexpr int kMaxEnt = 3;   // np <= 3 * kThreads
-
-struct Split {
-    int k0, lane, L, nent;   // nent: pairs this thread owns (0..kMaxEnt)
-};
-
-__device__ inline Split make_split(int np) {
-    Split sp;
-    const int t = threadIdx.x;
-    if (np <= 0) {
-        sp.k0 = 0; sp.lane = 0; sp.L = 1; sp.nent = 0;
-    } else if (np <= kThreads) {
-        sp.L = kThreads / np;
-        sp.k0 = t % np;
-        sp.lane = t / np;
-        sp.nent = sp.lane < sp.L ? 1 : 0;
-    } else {
-        sp.L = 1; sp.lane = 0; sp.k0 = t;
-        sp.nent = (np - t + kThreads - 1) / kThreads;
-        if (sp.nent > kMaxEnt) sp.nent = kMaxEnt;
-    }
-    return sp;
-}
-
-// Op pipeline.  Ops run in batches of kBatch.  Descriptors of batch b+1 sit
-// in an LDS ring (written one batch ahead, so reading them needs no global
-// load); each thread's pair entries for batch b+1 are loaded into registers
-// while batch b runs and handed over at the batch boundary, so their latency
-// overlaps a whole batch.  No scalar loads happen inside the loop: scalar and
-// LDS traffic share one wait counter.
+// Op pipeline.  Ops run in batches of kBatch.  Descriptors and term
+// coefficients of batch b+1 are staged in LDS rings while batch b runs, so no
+// global or scalar load sits between two ops (scalar and LDS traffic share one
+// wait counter).  Each op's schedule entries (kSchedMax loads per thread, a
+// static count so the compiler's wait counts stay exact) are loaded while the
+// previous op runs, into two alternating register sets.
 constexpr int kBatch = 8;
-constexpr int kWaitVmcnt0 = 0x0F70;   // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15) (gfx9 encoding)
 constexpr int kDescWords = sizeof(OpDev) / 4;   // 8
+constexpr int kChunk = 8;                       // lines (column pairs) whose LDS loads are issued together
+static_assert(kThreads == kThreadsPlan, "schedules are laid out for the pass kernel's thread count");
 
-struct BatchEnt {
-    uint32_t e[kBatch][kMaxEnt];
+// Schedule entries of one op for this thread.  The op's ns slots each get
+// L = kThreads / ns threads: thread t serves slot t % ns as lane t / ns and
+// takes lines (column pairs) lane, lane + L, ...
+struct EntSet {
+    uint32_t e[kSchedMax];
+    int slot, lane, L;
 };
 
-__device__ inline const OpDev &desc_at(const OpDev *ring, int slot, int i) { return ring[slot * kBatch + i]; }
+__device__ inline uint32_t s_rank1(uint32_t e) { return e & kSchedPad; }
+__device__ inline uint32_t s_rank2(uint32_t e) { return (e >> kSchedRankBits) & kSchedPad; }
+__device__ inline uint32_t s_slot(uint32_t e) { return (e >> 28) & 7u; }
+__device__ inline uint32_t s_par(uint32_t e) { return e >> 31; }
 
-// stage the descriptors of batch b (ops o0 + b*kBatch ...) into ring slot b % 2
-__device__ inline void stage_descs(const PassArgs &A, OpDev *ring, int o0, int o1, int b) {
+// stage descriptors (threads 0..63) and coefficients (threads 256..319) of batch b
+__device__ inline void stage_batch(const PassArgs &A, OpDev *ring, double2 *cring, int o0, int o1, int b) {
     const int first = o0 + b * kBatch;
     const int w = threadIdx.x;
     if (w < kBatch * kDescWords) {
@@ -142,34 +122,183 @@ __device__ inline void stage_descs(const PassArgs &A, OpDev *ring, int o0, int o
         uint32_t v = 0;
         if (first + i < o1) v = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
         reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[w] = v;
+    } else if (w >= 256 && w < 256 + kBatch * kMaxGroup) {
+        const int i = (w - 256) / kMaxGroup, slot = (w - 256) - i * kMaxGroup;
+        double2 c = make_double2(0.0, 0.0);
+        if (first + i < o1) {
+            const OpDev &od = A.ops[first + i];
+            if (od.type >= OP_AAAA && slot < od.g) {
+                const double *cf = A.coef + 2ull * (od.index + slot);
+                c = make_double2(cf[0], cf[1]);
+            }
+        }
+        cring[((b & 1) * kBatch + i) * kMaxGroup + slot] = c;
     }
 }
 
-__device__ inline void load_batch(const PassArgs &A, const OpDev *ring, int o0, int o1, int b, const Split &ca,
-                                  const Split &cc, BatchEnt &E) {
+__device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool valid, EntSet &E) {
+    const uint32_t *base = A.sched;
+    int count = 0, stride = 0;
+    E.slot = 0; E.lane = 1; E.L = 1;   // no work unless the op has a schedule
+    if (valid && d.type != OP_DIAG) {
+        const int ns = d.ns;
+        E.lane = (int)threadIdx.x / ns;
+        E.slot = (int)threadIdx.x - E.lane * ns;
+        E.L = kThreads / ns;
+        if (E.lane < E.L) {
+            base = A.sched + d.tab + E.slot;
+            stride = ns;
+            count = d.lg;
+        }
+    }
 #pragma unroll
-    for (int i = 0; i < kBatch; ++i) {
-        const int o = o0 + b * kBatch + i;
-        const OpDev &d = desc_at(ring, b & 1, i);
-        const int type = o < o1 ? d.type : -1;
-        const bool dbl = type == OP_AAAA || type == OP_BBBB;
-        const Split &sp = dbl ? cc : ca;
-        const uint32_t *tab = (dbl ? A.ccaa_table : A.ca_table) + d.tab;
+    for (int i = 0; i < kSchedMax; ++i) E.e[i] = base[(i < count ? i : 0) * stride];
+}
+
+__device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, int *pair_list, int *pair_count, int nl) {
+    const int tid = threadIdx.x;
+    const int q = (int)(qr & 0xff), r = (int)(qr >> 8);
+    if (tid < 64) {
+        bool is01 = false;
+        int partner = -1, parb = 0;
+        if (tid < nl) {
+            uint32_t s = line_str[tid];
+            if (((s >> r) & 1u) && !((s >> q) & 1u)) {
+                uint32_t ps = s ^ (1u << q) ^ (1u << r);
+                for (int l = 0; l < nl; ++l)
+                    if (line_str[l] == ps) partner = l;
+                is01 = partner >= 0;
+                parb = popcnt(s & between_mask(q, r)) & 1;
+            }
+        }
+        uint64_t mask = __ballot(is01);
+        if (is01) pair_list[__popcll(mask & ((1ull << tid) - 1ull))] = tid | (partner << 8) | (parb << 16);
+        if (tid == 0) *pair_count = __popcll(mask);
+    }
+}
+
+// rotate (line[r1], line[r2]) on lines lane, lane + L, ... of the tile with one (h, mu)
+template <typename R, typename V>
+__device__ inline void rotate_lines(V *tile, uint32_t C, int nl, int lane, int L, uint32_t r1, uint32_t r2, R h,
+                                    R mu) {
+    for (int l0 = lane; l0 < nl; l0 += kChunk * L) {
+        V u[kChunk], v[kChunk];
 #pragma unroll
-        for (int x = 0; x < kMaxEnt; ++x)
-            E.e[i][x] = (type > OP_DIAG && x < sp.nent) ? tab[sp.k0 + x * kThreads] : 0u;
+        for (int k = 0; k < kChunk; ++k)
+            if (l0 + k * L < nl) {
+                u[k] = tile[(size_t)(l0 + k * L) * C + r1];
+                v[k] = tile[(size_t)(l0 + k * L) * C + r2];
+            }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (l0 + k * L < nl) {
+                rotate(u[k], v[k], h, mu);
+                tile[(size_t)(l0 + k * L) * C + r1] = u[k];
+                tile[(size_t)(l0 + k * L) * C + r2] = v[k];
+            }
+    }
+}
+
+// rotate (l01[rs], l10[rp]) on column pairs lane, lane + L, ... of the pair list
+template <typename R, typename V>
+__device__ inline void rotate_pairs(V *tile, uint32_t C, const int *pair_list, int ncp, int lane, int L, uint32_t rs,
+                                    uint32_t rp, uint32_t par, R h, double sn) {
+    const R mpos = (R)sn, mneg = (R)(-sn);
+    for (int x0 = lane; x0 < ncp; x0 += kChunk * L) {
+        int pl[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) pl[k] = (x0 + k * L < ncp) ? pair_list[x0 + k * L] : 0;
+        V u[kChunk], v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (x0 + k * L < ncp) {
+                u[k] = tile[(size_t)(pl[k] & 0xff) * C + rs];
+                v[k] = tile[(size_t)((pl[k] >> 8) & 0xff) * C + rp];
+            }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (x0 + k * L < ncp) {
+                const R mu = ((par ^ (uint32_t)(pl[k] >> 16)) & 1u) ? mpos : mneg;
+                rotate(u[k], v[k], h, mu);
+                tile[(size_t)(pl[k] & 0xff) * C + rs] = u[k];
+                tile[(size_t)((pl[k] >> 8) & 0xff) * C + rp] = v[k];
+            }
+    }
+}
+
+// excitation group: the angle of pair (entry, line) is the pair part (ha, sa)
+// plus the line's other-string part (gang[l]), combined by angle addition in
+// the (1 - cos, sin) form
+template <typename R, typename V>
+__device__ inline void rotate_group(V *tile, uint32_t C, int nl, int lane, int L, uint32_t r1, uint32_t r2,
+                                    uint32_t par, double ha, double sa, const double2 *ga) {
+    for (int l0 = lane; l0 < nl; l0 += kChunk * L) {
+        V u[kChunk], v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (l0 + k * L < nl) {
+                u[k] = tile[(size_t)(l0 + k * L) * C + r1];
+                v[k] = tile[(size_t)(l0 + k * L) * C + r2];
+            }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (l0 + k * L < nl) {
+                const double2 b = ga[l0 + k * L];
+                const double h = (ha + b.x) - ha * b.x + sa * b.y;
+                const double s = (sa + b.y) - sa * b.x - ha * b.y;
+                rotate(u[k], v[k], (R)h, (R)(par ? s : -s));
+                tile[(size_t)(l0 + k * L) * C + r1] = u[k];
+                tile[(size_t)(l0 + k * L) * C + r2] = v[k];
+            }
     }
 }
 
 template <typename R>
-__device__ inline void run_op(const PassArgs &A, const OpDev &d, const uint32_t *ent, const Split &sca,
-                              const Split &scc, typename Vec<R>::type *tile, const double2 *gang,
-                              const uint32_t *line_rank, const uint32_t *line_str, int *pair_list, int *pair_count,
-                              int nl, uint32_t C, uint32_t &cur_qr, int &ncp) {
+__device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *coef, const EntSet &E,
+                              typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
+                              const uint32_t *line_str, int *pair_list, int *pair_count, int nl, uint32_t C,
+                              uint32_t &cur_qr, int &ncp) {
     using V = typename Vec<R>::type;
     const int tid = threadIdx.x;
     const int type = d.type;
-    if (type == OP_DIAG) {
+    if (type == OP_FAB) {
+        if (d.aux != cur_qr) {
+            cur_qr = d.aux;
+            build_pair_list(cur_qr, line_str, pair_list, pair_count, nl);
+            lds_barrier();
+            ncp = *pair_count;
+        }
+        if (E.lane >= E.L) return;
+#pragma unroll
+        for (int i = 0; i < kSchedMax; ++i) {
+            if (i >= d.lg) break;
+            const uint32_t e = E.e[i];
+            if (s_rank1(e) == kSchedPad) continue;
+            const double2 cf = coef[s_slot(e)];
+            rotate_pairs<R>(tile, C, pair_list, ncp, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), (R)cf.x, cf.y);
+        }
+    } else if (type == OP_FCC) {
+        if (E.lane >= E.L) return;
+#pragma unroll
+        for (int i = 0; i < kSchedMax; ++i) {
+            if (i >= d.lg) break;
+            const uint32_t e = E.e[i];
+            if (s_rank1(e) == kSchedPad) continue;
+            const double2 cf = coef[s_slot(e)];
+            rotate_lines<R>(tile, C, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x, (R)(s_par(e) ? cf.y : -cf.y));
+        }
+    } else if (type == OP_GA || type == OP_GB) {
+        if (E.lane >= E.L) return;
+        const double2 *ga = gang + (size_t)d.aux * nl;
+#pragma unroll
+        for (int i = 0; i < kSchedMax; ++i) {
+            if (i >= d.lg) break;
+            const uint32_t e = E.e[i];
+            if (s_rank1(e) == kSchedPad) continue;
+            const double *gp = A.grp_a + 2ull * ((size_t)d.index + (size_t)i * d.ns + E.slot);
+            rotate_group<R>(tile, C, nl, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), gp[0], gp[1], ga);
+        }
+    } else if (type == OP_DIAG) {
         const uint32_t total = (uint32_t)nl * C;
         for (uint32_t idx = tid; idx < total; idx += kThreads) {
             uint32_t l = idx / C, a = idx - l * C;
@@ -181,81 +310,124 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const uint32_t 
                 th += A.cross[(uint64_t)p * C + b];
                 sa &= sa - 1;
             }
-            double sn, cs;
-            sincos(th, &sn, &cs);
+            // e^{-i th} = 1 - h - i s with h = 2 sin^2(th/2), s = sin th
+            double s2, c2;
+            sincos(0.5 * th, &s2, &c2);
             V v = tile[idx], w;
-            R c = (R)cs, s = (R)(-sn);
-            w.x = c * v.x - s * v.y;
-            w.y = c * v.y + s * v.x;
+            const R h = (R)(2.0 * s2 * s2), s = (R)(2.0 * s2 * c2);
+            w.x = v.x + (s * v.y - h * v.x);
+            w.y = v.y + (-s * v.x - h * v.y);
             tile[idx] = w;
         }
-    } else if (type == OP_ABBA) {
-        if (d.aux != cur_qr) {
-            cur_qr = d.aux;
-            const int q = (int)(cur_qr & 0xff), r = (int)(cur_qr >> 8);
-            if (tid < 64) {
-                bool is01 = false;
-                int partner = -1, parb = 0;
-                if (tid < nl) {
-                    uint32_t s = line_str[tid];
-                    if (((s >> r) & 1u) && !((s >> q) & 1u)) {
-                        uint32_t ps = s ^ (1u << q) ^ (1u << r);
-                        for (int l = 0; l < nl; ++l)
-                            if (line_str[l] == ps) partner = l;
-                        is01 = partner >= 0;
-                        parb = popcnt(s & between_mask(q, r)) & 1;
-                    }
-                }
-                uint64_t mask = __ballot(is01);
-                if (is01) pair_list[__popcll(mask & ((1ull << tid) - 1ull))] = tid | (partner << 8) | (parb << 16);
-                if (tid == 0) *pair_count = __popcll(mask);
-            }
-            lds_barrier();
-            ncp = *pair_count;
-        }
-        const R c = (R)d.cs;
+    }
+}
+
+// Tile copies between HBM and LDS in 16-byte units, kCopyUnroll loads in
+// flight per thread before their LDS writes.  COL tiles: whole lines (columns)
+// of C amplitudes; the line and its LDS image are 16-byte aligned when
+// C * sizeof(V) is a multiple of 16, else the copy goes element by element.
+constexpr int kCopyUnroll = 8;
+
+// Every unrolled copy loads from a clamped (always valid) index so that the
+// staging array stays in registers; only the writes are guarded.
+template <typename V, typename U>
+__device__ inline void copy_cols_in(const V *src, V *tile, const uint32_t *line_rank, int nl, uint32_t C, double sc) {
+    const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));   // units per line
+    const uint32_t total = (uint32_t)nl * upl;
+    const U *s = reinterpret_cast<const U *>(src);
+    U *t = reinterpret_cast<U *>(tile);
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
+        U r[kCopyUnroll];
 #pragma unroll
-        for (int i = 0; i < kMaxEnt; ++i) {
-            if (i >= sca.nent) break;
-            const uint32_t e = ent[i];
-            const uint32_t rs_ = rank1(e), rp_ = rank2(e), para = par_of(e);
-            for (int x = sca.lane; x < ncp; x += sca.L) {
-                const int pl = pair_list[x];
-                V *l01 = tile + (size_t)(pl & 0xff) * C;
-                V *l10 = tile + (size_t)((pl >> 8) & 0xff) * C;
-                const R mu = (R)(((para ^ (uint32_t)(pl >> 16)) & 1u) ? d.sn : -d.sn);
-                rotate(l01[rs_], l10[rp_], c, mu);
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
+            const uint32_t l = i / upl, a = i - l * upl;
+            r[k] = s[(uint64_t)line_rank[l] * upl + a];
+        }
+#pragma unroll
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = i0 + (uint32_t)k * kThreads;
+            if (i < total) t[i] = r[k];
+        }
+    }
+    if (sc != 1.0) {
+        __syncthreads();
+        const uint32_t n = (uint32_t)nl * C;
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+            V v = tile[i];
+            v.x *= (decltype(v.x))sc;
+            v.y *= (decltype(v.y))sc;
+            tile[i] = v;
+        }
+    }
+}
+
+// ROW tiles: rows r0 .. r0+nl-1 (alpha strings) of all C columns, transposed
+// into lines of C amplitudes; global runs of nl consecutive amplitudes.
+template <typename V>
+__device__ inline void copy_rows_in(const V *src, V *tile, uint32_t r0, int nl, uint32_t C, double sc) {
+    const uint32_t total = (uint32_t)nl * C;
+    const auto f = (decltype(V().x))sc;
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
+        V r[kCopyUnroll];
+#pragma unroll
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
+            const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
+            r[k] = src[(uint64_t)b * C + r0 + l];
+        }
+#pragma unroll
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = i0 + (uint32_t)k * kThreads;
+            if (i < total) {
+                const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
+                V v = r[k];
+                v.x *= f;
+                v.y *= f;
+                tile[l * C + b] = v;
             }
         }
-    } else if (type == OP_GA || type == OP_GB) {
-        const double2 *ga = gang + (size_t)d.aux * nl;
-        const double *gp = A.grp_a + 2ull * d.index * A.ca_pairs;
+    }
+}
+
+template <typename V>
+__device__ inline void copy_rows_out(V *dst, const V *tile, uint32_t r0, int nl, uint32_t C) {
+    const uint32_t total = (uint32_t)nl * C;
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
+        V r[kCopyUnroll];
 #pragma unroll
-        for (int i = 0; i < kMaxEnt; ++i) {
-            if (i >= sca.nent) break;
-            const uint32_t e = ent[i];
-            const uint32_t rx = rank1(e), ry = rank2(e), par = par_of(e);
-            const int k = sca.k0 + i * kThreads;
-            const double ca = gp[2 * k], sa = gp[2 * k + 1];
-            for (int l = sca.lane; l < nl; l += sca.L) {
-                const double2 b = ga[l];
-                const double c = ca * b.x - sa * b.y;
-                const double s = sa * b.x + ca * b.y;
-                V *line = tile + (size_t)l * C;
-                rotate(line[rx], line[ry], (R)c, (R)(par ? s : -s));
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
+            const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
+            r[k] = tile[l * C + b];
+        }
+#pragma unroll
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = i0 + (uint32_t)k * kThreads;
+            if (i < total) {
+                const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
+                dst[(uint64_t)b * C + r0 + l] = r[k];
             }
         }
-    } else if (type >= 0) {   // AAAA / BBBB
-        const R c = (R)d.cs;
+    }
+}
+
+template <typename V, typename U>
+__device__ inline void copy_cols_out(V *dst, const V *tile, const uint32_t *line_rank, int nl, uint32_t C) {
+    const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));
+    const uint32_t total = (uint32_t)nl * upl;
+    U *d = reinterpret_cast<U *>(dst);
+    const U *t = reinterpret_cast<const U *>(tile);
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
+        U r[kCopyUnroll];
 #pragma unroll
-        for (int i = 0; i < kMaxEnt; ++i) {
-            if (i >= scc.nent) break;
-            const uint32_t e = ent[i];
-            const uint32_t r01 = rank1(e), r10 = rank2(e);
-            const R mu = (R)(par_of(e) ? d.sn : -d.sn);
-            for (int l = scc.lane; l < nl; l += scc.L) {
-                V *line = tile + (size_t)l * C;
-                rotate(line[r01], line[r10], c, mu);
+        for (int k = 0; k < kCopyUnroll; ++k) r[k] = t[min(i0 + (uint32_t)k * kThreads, total - 1)];
+#pragma unroll
+        for (int k = 0; k < kCopyUnroll; ++k) {
+            const uint32_t i = i0 + (uint32_t)k * kThreads;
+            if (i < total) {
+                const uint32_t l = i / upl, a = i - l * upl;
+                d[(uint64_t)line_rank[l] * upl + a] = r[k];
             }
         }
     }
@@ -271,7 +443,8 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines]
     int *pair_count = pair_list + kMaxLines;                            // [4]
     double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kWaves]
-    OpDev *ring = reinterpret_cast<OpDev *>(smem + 2048);              // [2][kBatch]
+    OpDev *ring = reinterpret_cast<OpDev *>(smem + 1024);              // [2][kBatch]
+    double2 *cring = reinterpret_cast<double2 *>(smem + 1536);          // [2][kBatch][kMaxGroup]
     V *tile = reinterpret_cast<V *>(smem + 4096);
 
     const int tid = threadIdx.x;
@@ -287,41 +460,27 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
         line_rank[tid] = r;
         line_str[tid] = A.strings[r];
     }
-    stage_descs(A, ring, o0, o1, 0);
+    stage_batch(A, ring, cring, o0, o1, 0);
     __syncthreads();
     for (int i = tid; i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
         const double *gb = A.grp_b + 2ull * (pass.group_first + g) * C + 2ull * line_rank[l];
         gang[i] = make_double2(gb[0], gb[1]);
     }
-    const Split sca = make_split((int)A.ca_pairs), scc = make_split((int)A.ccaa_pairs);
-    BatchEnt Ecur, Enext;
-    load_batch(A, ring, o0, o1, 0, sca, scc, Ecur);   // overlaps the tile load below
+    EntSet E0, E1;
+    load_entries(A, ring[0], o0 < o1, E0);   // overlaps the tile load below
 
     // ---- load the tile ---------------------------------------------------
     const bool from_prev = pass.flags & PF_FROM_PREV;
     const V *src = from_prev ? prev : cur;
-    R sc = (R)1;
-    if (from_prev && A.renorm) sc = (R)A.scale[j - 1];
+    const double sc = (from_prev && A.renorm) ? A.scale[j - 1] : 1.0;
     const uint32_t total = (uint32_t)nl * C;
+    const bool vec16 = (C * sizeof(V)) % 16 == 0;
     if (col) {
-        for (int l = 0; l < nl; ++l) {
-            const V *s_line = src + (uint64_t)line_rank[l] * C;
-            V *t_line = tile + (size_t)l * C;
-            for (uint32_t a = tid; a < C; a += kThreads) {
-                V v = s_line[a];
-                v.x *= sc; v.y *= sc;
-                t_line[a] = v;
-            }
-        }
+        if (vec16) copy_cols_in<V, uint4>(src, tile, line_rank, nl, C, sc);
+        else copy_cols_in<V, V>(src, tile, line_rank, nl, C, sc);
     } else {
-        const uint32_t r0 = it.begin;
-        for (uint32_t idx = tid; idx < total; idx += kThreads) {
-            uint32_t b = idx / nl, l = idx - b * nl;
-            V v = src[(uint64_t)b * C + r0 + l];
-            v.x *= sc; v.y *= sc;
-            tile[l * C + b] = v;
-        }
+        copy_rows_in<V>(src, tile, it.begin, nl, C, sc);
     }
 
     // ---- ops ------------------------------------------------------------------
@@ -329,23 +488,33 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     int ncp = 0;
     const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
     for (int b = 0; b < nbatch; ++b) {
-        // this batch's entries (loaded one batch ago) must land before the next
-        // batch's loads are issued: the vector-memory counter retires in order
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        // descriptors of batch b+1 into the other ring slot, then its entries
         __syncthreads();
-        if (b + 1 < nbatch) stage_descs(A, ring, o0, o1, b + 1);
+        if (b + 1 < nbatch) stage_batch(A, ring, cring, o0, o1, b + 1);
         __syncthreads();
-        if (b + 1 < nbatch) load_batch(A, ring, o0, o1, b + 1, sca, scc, Enext);
-#pragma unroll
-        for (int i = 0; i < kBatch; ++i) {
-            if (o0 + b * kBatch + i >= o1) break;
-            const OpDev d = desc_at(ring, b & 1, i);
+        // two ops per iteration: the even op runs on E0 while E1 loads, the odd op the other way round
+        for (int i = 0; i < kBatch; i += 2) {
+            const int o = o0 + b * kBatch + i;
+            if (o >= o1) break;
+            const OpDev *rb = ring + (b & 1) * kBatch;
+            const double2 *cb = cring + (size_t)((b & 1) * kBatch) * kMaxGroup;
+            load_entries(A, rb[i + 1], o + 1 < o1, E1);
             lds_barrier();
-            run_op<R>(A, d, Ecur.e[i], sca, scc, tile, gang, line_rank, line_str, pair_list, pair_count, nl, C,
-                      cur_qr, ncp);
+            {
+                const OpDev d = rb[i];
+                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, pair_list, pair_count, nl, C,
+                          cur_qr, ncp);
+            }
+            if (o + 1 >= o1) break;
+            // entries of op o+2 (the next batch's first op after the last pair)
+            const OpDev &dn = (i + 2 < kBatch) ? rb[i + 2] : ring[((b + 1) & 1) * kBatch];
+            load_entries(A, dn, o + 2 < o1, E0);
+            lds_barrier();
+            {
+                const OpDev d = rb[i + 1];
+                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, pair_list, pair_count,
+                          nl, C, cur_qr, ncp);
+            }
         }
-        Ecur = Enext;
     }
     __syncthreads();
 
@@ -378,17 +547,10 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
 
     // ---- store the tile -----------------------------------------------------
     if (col) {
-        for (int l = 0; l < nl; ++l) {
-            V *d_line = cur + (uint64_t)line_rank[l] * C;
-            const V *t_line = tile + (size_t)l * C;
-            for (uint32_t a = tid; a < C; a += kThreads) d_line[a] = t_line[a];
-        }
+        if (vec16) copy_cols_out<V, uint4>(cur, tile, line_rank, nl, C);
+        else copy_cols_out<V, V>(cur, tile, line_rank, nl, C);
     } else {
-        const uint32_t r0 = it.begin;
-        for (uint32_t idx = tid; idx < total; idx += kThreads) {
-            uint32_t b = idx / nl, l = idx - b * nl;
-            cur[(uint64_t)b * C + r0 + l] = tile[l * C + b];
-        }
+        copy_rows_out<V>(cur, tile, it.begin, nl, C);
     }
 }
 
@@ -652,12 +814,11 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     to_device(&groups, p.groups, s);
     to_device(&grp_a, p.grp_a, s);
     to_device(&grp_b, p.grp_b, s);
-    to_device(&terms, p.terms, s);
     to_device(&ops, p.ops, s);
     to_device(&items, p.items, s);
     to_device(&col_list, p.col_list, s);
-    to_device(&ca_table, p.ca_table, s);
-    to_device(&ccaa_table, p.ccaa_table, s);
+    to_device(&sched, p.sched, s);
+    to_device(&coef, p.coef, s);
     hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
     lds_bytes = 0;
     for (const auto &ps : p.passes) {
@@ -669,13 +830,14 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
 }
 
 void DevicePlan::release() {
-    void *ptrs[] = {binom, strings, theta_a, theta_b, cross, groups, grp_a, grp_b, terms, ops, items, col_list,
-                    partials, ca_table, ccaa_table};
+    void *ptrs[] = {binom, strings, theta_a, theta_b, cross, groups, grp_a, grp_b, ops, items, col_list,
+                    partials, sched, coef};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     binom = nullptr; strings = nullptr; theta_a = theta_b = cross = nullptr; groups = nullptr;
-    grp_a = grp_b = nullptr; terms = nullptr; ops = nullptr; items = nullptr; col_list = nullptr; partials = nullptr;
-    ca_table = ccaa_table = nullptr;
+    grp_a = grp_b = nullptr; ops = nullptr; items = nullptr; col_list = nullptr; partials = nullptr;
+    sched = nullptr;
+    coef = nullptr;
 }
 
 template <typename R>
@@ -687,8 +849,8 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norm
     A.binom = dp.binom; A.strings = dp.strings;
     A.theta_a = dp.theta_a; A.theta_b = dp.theta_b; A.cross = dp.cross;
     A.groups = dp.groups; A.grp_a = dp.grp_a; A.grp_b = dp.grp_b;
-    A.terms = dp.terms; A.ops = dp.ops; A.items = dp.items; A.col_list = dp.col_list;
-    A.ca_table = dp.ca_table; A.ccaa_table = dp.ccaa_table;
+    A.ops = dp.ops; A.items = dp.items; A.col_list = dp.col_list;
+    A.sched = dp.sched; A.coef = dp.coef;
     A.partials = dp.partials; A.scale = scale;
     A.n = p.n; A.e = p.e; A.C = p.C; A.ca_pairs = p.ca_pairs; A.ccaa_pairs = p.ccaa_pairs;
     A.renorm = renorm;

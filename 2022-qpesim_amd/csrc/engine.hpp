@@ -21,11 +21,11 @@ struct DevicePlan {
     double *theta_a = nullptr, *theta_b = nullptr, *cross = nullptr;
     GroupDev *groups = nullptr;
     double *grp_a = nullptr, *grp_b = nullptr;
-    TermDev *terms = nullptr;
     OpDev *ops = nullptr;
     ItemDev *items = nullptr;
     uint32_t *col_list = nullptr;
-    uint32_t *ca_table = nullptr, *ccaa_table = nullptr;
+    uint32_t *sched = nullptr;
+    double *coef = nullptr;
     double *partials = nullptr;   // [max_items] norm partial sums
     size_t lds_bytes = 0;         // dynamic LDS per pass launch
 

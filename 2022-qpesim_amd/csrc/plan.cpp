@@ -34,7 +34,7 @@ uint32_t expand_outside(uint32_t y, const std::vector<int> &tpos_sorted) {
 
 }  // namespace
 
-Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse) {
     if (!h.indexed()) throw std::runtime_error("IndexTerms has not been called");
     Plan P;
     const int n = static_cast<int>(h.num_orbitals());
@@ -55,9 +55,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     P.strings.resize(C);
     for (uint32_t i = 0; i < C; ++i) P.strings[i] = unrank_of(i, e, n, B.tab);
 
-    if (C > kRankMask) throw std::runtime_error("C(num_orbitals, num_occupied) exceeds the packed-rank range");
-    if (P.ca_pairs > kMaxPairs || P.ccaa_pairs > kMaxPairs)
-        throw std::runtime_error("more determinant pairs per term than the tile engine unrolls");
+    if (C > kSchedPad) throw std::runtime_error("C(num_orbitals, num_occupied) exceeds the 14-bit rank range of schedule entries");
     // single-excitation pair tables for every ordered orbital pair (x, y)
     P.ca_table.assign((size_t)n * n * P.ca_pairs, 0u);
     for (int x = 0; x < n; ++x)
@@ -122,7 +120,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     std::map<std::pair<int, int>, std::vector<const IndexedTerm *>> gated;  // (p_, r_) -> pqqr terms
     for (const auto &T : seg[kPQQR]) gated[{T.idx[0], T.idx[3]}].push_back(&T);
     const uint32_t npa = P.ca_pairs;
-    std::vector<int> group_spin;
+    std::vector<double> grp_pair;   // [G][npa][2] 1-cos, sin of the pair angle part, pair order k
     for (const auto &T : seg[kPQ]) {
         int gp = T.idx[0], gq = T.idx[1];   // gp > gq, same spin
         int sp = spin_of(gp);
@@ -148,8 +146,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             }
         }
         // pair part: t*(h + sum of same-spin inner coefficients occupied in f)
-        size_t base_a = P.grp_a.size();
-        P.grp_a.resize(base_a + 2 * (size_t)npa);
+        size_t base_a = grp_pair.size();
+        grp_pair.resize(base_a + 2 * (size_t)npa);
         for (uint32_t k = 0; k < npa; ++k) {
             uint32_t r1, r2, par;
             uint32_t f = ca_pair(k, g.x, g.y, n, e, B.tab, &r1, &r2, &par);
@@ -157,8 +155,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             for (int x = 0; x < n; ++x)
                 if (f & (1u << x)) ang += csame[x];
             ang *= t;
-            P.grp_a[base_a + 2 * k] = std::cos(ang);
-            P.grp_a[base_a + 2 * k + 1] = std::sin(ang);
+            grp_pair[base_a + 2 * k] = versine(ang);
+            grp_pair[base_a + 2 * k + 1] = std::sin(ang);
         }
         // other-string part: t*(sum of opposite-spin inner coefficients occupied)
         size_t base_b = P.grp_b.size();
@@ -168,7 +166,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             for (int y = 0; y < n; ++y)
                 if (P.strings[i] & (1u << y)) ang += cother[y];
             ang *= t;
-            P.grp_b[base_b + 2 * i] = std::cos(ang);
+            P.grp_b[base_b + 2 * i] = versine(ang);
             P.grp_b[base_b + 2 * i + 1] = std::sin(ang);
         }
         P.groups.push_back(g);
@@ -177,7 +175,6 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     // ---- pqrs terms (TEOT:519-560) --------------------------------------
     struct Rot { int cls; int index; int p, q, r, s; };
     std::vector<Rot> rots;
-    std::vector<uint32_t> cc_offset;   // per term: offset in ccaa_table
     for (const auto &T : seg[kPQRS]) {
         int p = T.idx[0], q = T.idx[1], r = T.idx[2], s = T.idx[3];
         int cls;
@@ -191,18 +188,9 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         d.cs = std::cos(a);
         d.sn = std::sin(a);
         rots.push_back({cls, (int)P.terms.size(), d.p, d.q, d.r, d.s});
-        if (cls != OP_ABBA) {
-            uint32_t base = (uint32_t)P.ccaa_table.size();
-            cc_offset.push_back(base);
-            for (uint32_t k = 0; k < P.ccaa_pairs; ++k) {
-                uint32_t r01, r10, par;
-                ccaa_pair(k, d.p, d.q, d.r, d.s, n, e, B.tab, &r01, &r10, &par);
-                P.ccaa_table.push_back(pack_pair(r01, r10, par));
-            }
-        } else {
-            cc_offset.push_back(0);
-        }
         P.terms.push_back(d);
+        P.coef.push_back(versine(a));
+        P.coef.push_back(d.sn);
         P.pair_updates += (cls == OP_ABBA) ? (uint64_t)npa * npa : (uint64_t)P.ccaa_pairs * C;
     }
 
@@ -215,32 +203,293 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
     };
     const int W = lines_for(0);
     if (W < 1) throw std::runtime_error("a single column does not fit the tile budget");
-    const uint32_t nca = P.ca_pairs;
-    auto ca_off = [&](int x, int y) { return (uint32_t)(((size_t)x * n + y) * nca); };
-    auto op_group = [&](int type, int g, int slot) {
+    const int NS = kThreadsPlan;   // schedule slots: one per thread of the pass kernel
+    const size_t max_group = fuse ? (size_t)kMaxGroup : 1u;
+    auto blank = [](int type) {
         OpDev o;
-        o.type = type;
-        o.index = g;
-        o.tab = ca_off(P.groups[g].x, P.groups[g].y);
-        o.aux = (uint32_t)slot;
-        o.cs = 1.0;
-        o.sn = 0.0;
+        o.type = type; o.index = 0; o.tab = 0; o.aux = 0; o.lg = 0; o.ns = kThreadsPlan; o.g = 1; o.pad = 0;
         return o;
     };
-    auto op_term = [&](const Rot &R) {
-        OpDev o;
-        o.type = R.cls;
-        o.index = R.index;
-        if (R.cls == OP_ABBA) {
-            o.tab = ca_off(R.s, R.p);        // alpha pair: rank(f|s), rank(f|p)
-            o.aux = (uint32_t)R.q | ((uint32_t)R.r << 8);
-        } else {
-            o.tab = cc_offset[R.index];
-            o.aux = 0;
+    const uint32_t pad_entry = pack_sched(kSchedPad, kSchedPad, 0, 0);
+    // Appends rows [i0, i0+lg) of slot lists as a schedule: entry i of slot t at tab + i*ns + t.
+    using Slots = std::vector<std::vector<uint32_t>>;
+    auto emit_rows = [&](const Slots &slot, int i0, int lg, OpDev &o) {
+        const int ns = (int)slot.size();
+        o.tab = (uint32_t)P.sched.size();
+        o.lg = lg;
+        o.ns = ns;
+        for (int i = i0; i < i0 + lg; ++i)
+            for (int k = 0; k < ns; ++k)
+                P.sched.push_back((size_t)i < slot[k].size() ? slot[k][i] : pad_entry);
+    };
+    auto slot_len = [](const Slots &slot) {
+        size_t lg = 0;
+        for (auto &sl : slot) lg = std::max(lg, sl.size());
+        return (int)lg;
+    };
+    // Work split of an op: ns schedule slots; the L = NS / ns threads of a slot
+    // share its lines (or column pairs).  Cost model: rows x units per thread.
+    auto split_cost = [&](int lg, int ns, int units) {
+        const int L = NS / ns;
+        return (double)lg * (double)((units + L - 1) / L) + 0.01 * lg;
+    };
+    // excitation group g: its pairs k in slot k % ns, row k / ns, split into
+    // ops of at most kSchedMax rows; the pair angles are laid out the same way
+    auto push_group_ops = [&](int type, int g, int stage_slot, int units) {
+        (void)units;
+        const GroupDev &gd = P.groups[g];
+        // the pair angles are read inside the op (one global latency per row): fewest rows
+        const int ns = (int)std::min<uint32_t>(NS, npa);
+        const int rows = (int)((npa + ns - 1) / ns);
+        for (int i0 = 0; i0 < rows; i0 += kSchedMax) {
+            const int lg = std::min(kSchedMax, rows - i0);
+            OpDev o = blank(type);
+            o.aux = (uint32_t)stage_slot;
+            o.index = (int32_t)(P.grp_a.size() / 2);
+            o.tab = (uint32_t)P.sched.size();
+            o.lg = lg;
+            o.ns = ns;
+            for (int i = i0; i < i0 + lg; ++i)
+                for (int k = 0; k < ns; ++k) {
+                    const size_t kk = (size_t)i * ns + k;
+                    if (kk < npa) {
+                        const uint32_t e0 = P.ca_table[((size_t)gd.x * n + gd.y) * npa + kk];
+                        P.sched.push_back(pack_sched(e0 & kRankMask, (e0 >> kRankBits) & kRankMask, 0, e0 >> 31));
+                        P.grp_a.push_back(grp_pair[2 * ((size_t)g * npa + kk)]);
+                        P.grp_a.push_back(grp_pair[2 * ((size_t)g * npa + kk) + 1]);
+                    } else {
+                        P.sched.push_back(pad_entry);
+                        P.grp_a.push_back(0.0);
+                        P.grp_a.push_back(0.0);
+                    }
+                }
+            P.ops.push_back(o);
         }
-        o.cs = P.terms[R.index].cs;
-        o.sn = P.terms[R.index].sn;
-        return o;
+    };
+    // Packs per-orbit entry lists into ns slots, longest list first onto the
+    // least-loaded slot (a min-heap over slot loads).  pack_rows returns only
+    // the resulting schedule length; pack_slots builds the slot lists.
+    using Orbits = std::vector<std::vector<uint32_t>>;
+    auto lpt_order = [](const Orbits &orbits) {
+        std::vector<uint32_t> idx(orbits.size());
+        for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](uint32_t a, uint32_t b) { return orbits[a].size() > orbits[b].size(); });
+        return idx;
+    };
+    auto lpt_assign = [](const Orbits &orbits, const std::vector<uint32_t> &order, int ns, std::vector<int> *where) {
+        bool unit = true;
+        for (const auto &o : orbits) unit = unit && o.size() == 1;
+        if (unit) {   // one entry per orbit: round robin is the LPT packing
+            if (where) {
+                where->assign(orbits.size(), 0);
+                for (size_t r = 0; r < order.size(); ++r) (*where)[order[r]] = (int)(r % ns);
+            }
+            return (int)((orbits.size() + ns - 1) / ns);
+        }
+        std::vector<std::pair<size_t, int>> heap;   // (load, slot), min-heap
+        heap.reserve(ns);
+        for (int k = 0; k < ns; ++k) heap.push_back({0, k});
+        auto cmp = [](const std::pair<size_t, int> &a, const std::pair<size_t, int> &b) { return a > b; };
+        std::make_heap(heap.begin(), heap.end(), cmp);
+        size_t lg = 0;
+        if (where) where->assign(orbits.size(), 0);
+        for (uint32_t oi : order) {
+            std::pop_heap(heap.begin(), heap.end(), cmp);
+            auto &top = heap.back();
+            if (where) (*where)[oi] = top.second;
+            top.first += orbits[oi].size();
+            lg = std::max(lg, top.first);
+            std::push_heap(heap.begin(), heap.end(), cmp);
+        }
+        return (int)lg;
+    };
+    auto pack_slots = [&](const Orbits &orbits, const std::vector<uint32_t> &order, int ns) {
+        std::vector<int> where;
+        lpt_assign(orbits, order, ns, &where);
+        Slots slot(ns);
+        for (uint32_t oi : order) slot[where[oi]].insert(slot[where[oi]].end(), orbits[oi].begin(), orbits[oi].end());
+        return slot;
+    };
+    // orbits of a support set S (positions spos) inside one string: callback(y, kS)
+    auto for_orbits = [&](const std::vector<int> &spos, auto &&fn) {
+        const int ts = (int)spos.size();
+        for (int k = std::max(0, e - (n - ts)); k <= std::min(e, ts); ++k) {
+            uint32_t ny = (uint32_t)B(n - ts, e - k);
+            for (uint32_t yi = 0; yi < ny; ++yi) fn(expand_outside(unrank_of(yi, e - k, n - ts, B.tab), spos), k);
+        }
+    };
+    auto members = [&](const std::vector<int> &spos, int k) {
+        std::vector<uint32_t> m;
+        const int ts = (int)spos.size();
+        for (uint32_t pi = 0; pi < (uint32_t)B(ts, k); ++pi) m.push_back(scatter(unrank_of(pi, k, ts, B.tab), spos));
+        return m;
+    };
+    auto positions = [&](uint32_t mask) {
+        std::vector<int> v;
+        for (int b = 0; b < n; ++b)
+            if (mask & (1u << b)) v.push_back(b);
+        return v;
+    };
+    // Per-orbit entries of consecutive same-string double excitations T[0..g):
+    // an orbit is the set of strings that agree outside the joint support, and
+    // it is closed under every term of the group, so one thread applies the
+    // group's rotations of an orbit in order without synchronising.
+    auto cc_orbits = [&](const std::vector<Rot> &T) {
+        uint32_t smask = 0;
+        for (const auto &R : T) smask |= (1u << R.p) | (1u << R.q) | (1u << R.r) | (1u << R.s);
+        std::vector<int> spos = positions(smask);
+        Orbits orbits;
+        std::vector<std::vector<uint32_t>> mem_k(spos.size() + 1);
+        for (size_t k = 0; k <= spos.size(); ++k) mem_k[k] = members(spos, (int)k);
+        for_orbits(spos, [&](uint32_t y, int k) {
+            std::vector<uint32_t> ent;
+            const auto &mem = mem_k[k];
+            for (size_t ti = 0; ti < T.size(); ++ti) {
+                const Rot &R = T[ti];
+                const uint32_t from = (1u << R.r) | (1u << R.s), to = (1u << R.p) | (1u << R.q);
+                int id[4] = {R.p, R.q, R.r, R.s};
+                std::sort(id, id + 4);
+                for (uint32_t m : mem) {
+                    uint32_t a = y | m;
+                    if ((a & from) != from || (a & to)) continue;
+                    uint32_t b = a ^ from ^ to, f = a ^ from;
+                    uint32_t par = (uint32_t)(popcnt(f & between_mask(id[0], id[1])) + popcnt(f & between_mask(id[2], id[3]))) & 1u;
+                    ent.push_back(pack_sched(rank_of(a, B.tab), rank_of(b, B.tab), (uint32_t)ti, par));
+                }
+            }
+            if (!ent.empty()) orbits.push_back(std::move(ent));
+        });
+        return orbits;
+    };
+    // consecutive ABBA terms (p, q, r, s_i): alpha orbits over {p} + {s_i}
+    auto ab_orbits = [&](const std::vector<Rot> &T) {
+        const int p = T[0].p;
+        uint32_t smask = 1u << p;
+        for (const auto &R : T) smask |= 1u << R.s;
+        std::vector<int> spos = positions(smask);
+        Orbits orbits;
+        std::vector<std::vector<uint32_t>> mem_k(spos.size() + 1);
+        for (size_t k = 0; k <= spos.size(); ++k) mem_k[k] = members(spos, (int)k);
+        for_orbits(spos, [&](uint32_t y, int k) {
+            std::vector<uint32_t> ent;
+            const auto &mem = mem_k[k];
+            for (size_t ti = 0; ti < T.size(); ++ti) {
+                const int sv = T[ti].s;
+                for (uint32_t m : mem) {
+                    uint32_t a = y | m;
+                    if (!((a >> sv) & 1u) || ((a >> p) & 1u)) continue;
+                    uint32_t b = a ^ (1u << sv) ^ (1u << p), f = a ^ (1u << sv);
+                    uint32_t par = (uint32_t)popcnt(f & between_mask(sv, p)) & 1u;
+                    ent.push_back(pack_sched(rank_of(a, B.tab), rank_of(b, B.tab), (uint32_t)ti, par));
+                }
+            }
+            if (!ent.empty()) orbits.push_back(std::move(ent));
+        });
+        return orbits;
+    };
+    // a single term: every determinant pair is its own orbit
+    auto single_orbits = [&](const Rot &R, bool ab) {
+        Orbits ob;
+        if (ab) {
+            const size_t base = ((size_t)R.s * n + R.p) * npa;   // pairs rank(f|s), rank(f|p)
+            for (uint32_t k = 0; k < npa; ++k) {
+                const uint32_t e0 = P.ca_table[base + k];
+                ob.push_back({pack_sched(e0 & kRankMask, (e0 >> kRankBits) & kRankMask, 0, e0 >> 31)});
+            }
+        } else {
+            for (uint32_t k = 0; k < P.ccaa_pairs; ++k) {
+                uint32_t r01, r10, par;
+                ccaa_pair(k, R.p, R.q, R.r, R.s, n, e, B.tab, &r01, &r10, &par);
+                ob.push_back({pack_sched(r01, r10, 0, par)});
+            }
+        }
+        return ob;
+    };
+    auto orbit_stats = [](const Orbits &ob, size_t &total, size_t &longest) {
+        total = 0;
+        longest = 0;
+        for (auto &o : ob) {
+            total += o.size();
+            longest = std::max(longest, o.size());
+        }
+    };
+    // Greedy grouping of consecutive terms: a group grows while its joint
+    // support stays within the limit, no orbit is longer than kSchedMax and
+    // the entries fill at most 3/4 of the NS x kSchedMax schedule (so the
+    // packing fits).  A single term's entries are independent: a term with more
+    // than NS x kSchedMax of them is split over several ops.
+    auto push_groups = [&](const std::vector<Rot> &list, bool ab, int units) {
+        auto support = [&](const Rot &R) {
+            return ab ? (1u << R.p) | (1u << R.s) : (1u << R.p) | (1u << R.q) | (1u << R.r) | (1u << R.s);
+        };
+        const int limit = ab ? kFuseSupportAB : kFuseSupportCC;
+        const size_t cap = (size_t)NS * kSchedMax * 3 / 4;
+        size_t i = 0;
+        while (i < list.size()) {
+            std::vector<Rot> grp{list[i]};
+            Orbits best = single_orbits(list[i], ab);
+            uint32_t sm = support(list[i]);
+            size_t j = i + 1;
+            // coefficients of a group are read as coef[index + slot]: term indices must be consecutive
+            while (j < list.size() && grp.size() < max_group && list[j].index == grp[0].index + (int)grp.size()) {
+                uint32_t nm = sm | support(list[j]);
+                if (popcnt(nm) > limit) break;
+                // closed forms: an orbit with k of the |S| support orbitals occupied holds, per
+                // term, C(|S|-2, k-1) ABBA entries (s occupied, p empty) or C(|S|-4, k-2)
+                // same-string entries (r, s occupied, p, q empty)
+                const int ts = popcnt(nm), gg = (int)grp.size() + 1;
+                uint64_t longest = 0;
+                for (int k = std::max(0, e - (n - ts)); k <= std::min(e, ts); ++k) {
+                    const uint64_t per = ab ? (k >= 1 ? B(ts - 2, k - 1) : 0) : (k >= 2 ? B(ts - 4, k - 2) : 0);
+                    longest = std::max<uint64_t>(longest, per * gg);
+                }
+                const uint64_t total = (uint64_t)gg * (ab ? P.ca_pairs : P.ccaa_pairs);
+                if (longest > (uint64_t)kSchedMax || total > cap) break;
+                grp.push_back(list[j]);
+                sm = nm;
+                ++j;
+            }
+            if (grp.size() > 1) best = ab ? ab_orbits(grp) : cc_orbits(grp);
+            // slot count with the lowest cost among those whose schedule fits kSchedMax rows
+            // (a single term may exceed it at any slot count: its rows are then split over ops)
+            std::vector<uint32_t> order = lpt_order(best);
+            int lg512 = lpt_assign(best, order, NS, nullptr);
+            while (grp.size() > 1 && lg512 > kSchedMax) {   // the packing did not fit: shrink
+                grp.pop_back();
+                --j;
+                best = grp.size() > 1 ? (ab ? ab_orbits(grp) : cc_orbits(grp)) : single_orbits(grp[0], ab);
+                order = lpt_order(best);
+                lg512 = lpt_assign(best, order, NS, nullptr);
+            }
+            // slot count with the lowest cost among those whose schedule fits kSchedMax rows
+            // (a single term may exceed it at any slot count: its rows are then split over ops)
+            int ns_best = NS;
+            double bc = split_cost(std::min(lg512, kSchedMax), NS, units) * ((lg512 + kSchedMax - 1) / kSchedMax);
+            size_t total = 0, longest = 0;
+            orbit_stats(best, total, longest);
+            for (int L = 2; L <= std::min(NS, units); ++L) {   // more lanes than units idle
+                const int cand = NS / L;
+                if ((size_t)cand * kSchedMax < total || longest > (size_t)kSchedMax) break;
+                const int lower = (int)std::max<size_t>(longest, (total + cand - 1) / cand);
+                if (split_cost(lower, cand, units) >= bc - 1e-9) continue;
+                const int lgc = lpt_assign(best, order, cand, nullptr);
+                if (lgc > kSchedMax) continue;
+                const double c = split_cost(lgc, cand, units);
+                if (c < bc - 1e-9) { bc = c; ns_best = cand; }
+            }
+            Slots slot = pack_slots(best, order, ns_best);
+            const int rows = slot_len(slot);
+            for (int i0 = 0; i0 < std::max(rows, 1); i0 += kSchedMax) {
+                OpDev o = blank(ab ? OP_FAB : OP_FCC);
+                o.index = grp[0].index;
+                o.g = (int)grp.size();
+                o.aux = ab ? ((uint32_t)grp[0].q | ((uint32_t)grp[0].r << 8)) : 0u;
+                emit_rows(slot, i0, std::min(kSchedMax, rows - i0), o);
+                P.ops.push_back(o);
+            }
+            i = j;
+        }
     };
 
     auto add_pass = [&](int kind, int flags, int op_begin, std::vector<ItemDev> items, int max_lines,
@@ -291,10 +540,8 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         int w = lines_for(ga_count);
         if (w < 1) throw std::runtime_error("tile budget too small for the alpha excitation groups");
         int ob = (int)P.ops.size();
-        OpDev d;
-        d.type = OP_DIAG; d.index = 0; d.tab = 0; d.aux = 0; d.cs = 1.0; d.sn = 0.0;
-        P.ops.push_back(d);
-        for (int g = 0; g < ga_count; ++g) P.ops.push_back(op_group(OP_GA, ga_first + g, g));
+        P.ops.push_back(blank(OP_DIAG));
+        for (int g = 0; g < ga_count; ++g) push_group_ops(OP_GA, ga_first + g, g, w);
         add_pass(PASS_COL, PF_FROM_PREV, ob, chunk_columns(w), w, ga_count ? ga_first : 0, ga_count);
     }
     // pass 1: beta groups (ROW)
@@ -302,18 +549,14 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
         int w = lines_for(gb_count);
         if (w < 1) throw std::runtime_error("tile budget too small for the beta excitation groups");
         int ob = (int)P.ops.size();
-        for (int g = 0; g < gb_count; ++g) P.ops.push_back(op_group(OP_GB, gb_first + g, g));
+        for (int g = 0; g < gb_count; ++g) push_group_ops(OP_GB, gb_first + g, g, w);
         add_pass(PASS_ROW, 0, ob, chunk_rows(w), w, gb_first, gb_count);
     }
-    // alpha-leading double excitations, p by p: AAAA_p then ABBA_p (map order)
+    // alpha-leading double excitations, p by p: AAAA_p (COL, all columns) then ABBA_p windows
     auto max_orbit = [&](int tsize, int k0, int k1) {
         uint64_t m = 0;
         for (int k = k0; k <= k1; ++k) m = std::max<uint64_t>(m, B(tsize, k));
         return m;
-    };
-    auto k_range = [&](int tsize, bool all, int *k0, int *k1) {
-        *k0 = std::max(all ? 0 : 1, e - (n - tsize));
-        *k1 = std::min(e, all ? tsize : tsize - 1);
     };
     size_t ri = 0;
     while (ri < rots.size() && rots[ri].cls != OP_BBBB) {
@@ -323,7 +566,11 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             (rots[ri].cls == OP_AAAA ? aaaa : abba).push_back(rots[ri]);
             ++ri;
         }
-        bool pending_aaaa = !aaaa.empty();
+        if (!aaaa.empty()) {
+            int ob = (int)P.ops.size();
+            push_groups(aaaa, false, W);
+            add_pass(PASS_COL, 0, ob, chunk_columns(W), W, 0, 0);
+        }
         size_t ai = 0;
         while (ai < abba.size()) {
             // window: consecutive terms sharing q, over a run of r values
@@ -335,24 +582,23 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
                 uint32_t nm = tmask | (1u << r);
                 if (nm != tmask) {
                     int ts = popcnt(nm);
-                    int k0, k1;
-                    k_range(ts, pending_aaaa, &k0, &k1);
+                    int k0 = std::max(1, e - (n - ts)), k1 = std::min(e, ts - 1);
                     if (max_orbit(ts, k0, k1) > (uint64_t)W) break;
                     tmask = nm;
                 }
                 ++aj;
             }
             if (aj == ai) throw std::runtime_error("tile budget too small for a mixed double excitation");
-            std::vector<int> tpos;
-            for (int b = 0; b < n; ++b)
-                if (tmask & (1u << b)) tpos.push_back(b);
+            std::vector<int> tpos = positions(tmask);
             int ts = (int)tpos.size();
-            int k0, k1;
-            k_range(ts, pending_aaaa, &k0, &k1);
+            int k0 = std::max(1, e - (n - ts)), k1 = std::min(e, ts - 1);
             int ob = (int)P.ops.size();
-            if (pending_aaaa)
-                for (const auto &R : aaaa) P.ops.push_back(op_term(R));
-            for (size_t x = ai; x < aj; ++x) P.ops.push_back(op_term(abba[x]));
+            for (size_t x = ai; x < aj;) {
+                size_t y = x;
+                while (y < aj && abba[y].r == abba[x].r) ++y;
+                push_groups(std::vector<Rot>(abba.begin() + x, abba.begin() + y), true, std::max(1, W / 3));
+                x = y;
+            }
             // orbits: outside string y (e-k electrons on n-ts positions) x all k-patterns of T
             std::vector<ItemDev> items;
             ItemDev cur{(uint32_t)P.col_list.size(), 0};
@@ -373,21 +619,17 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes) {
             }
             if (cur.count) items.push_back(cur);
             add_pass(PASS_COL, 0, ob, items, W, 0, 0);
-            pending_aaaa = false;
             ai = aj;
-        }
-        if (pending_aaaa) {
-            int ob = (int)P.ops.size();
-            for (const auto &R : aaaa) P.ops.push_back(op_term(R));
-            add_pass(PASS_COL, 0, ob, chunk_columns(W), W, 0, 0);
         }
     }
     // pure-beta double excitations (ROW)
     {
         int ob = (int)P.ops.size();
-        for (; ri < rots.size(); ++ri) P.ops.push_back(op_term(rots[ri]));
+        std::vector<Rot> bbbb(rots.begin() + ri, rots.end());
+        push_groups(bbbb, false, W);
         if ((int)P.ops.size() > ob) add_pass(PASS_ROW, 0, ob, chunk_rows(W), W, 0, 0);
     }
+    if (P.sched.empty()) P.sched.push_back(pack_sched(kSchedPad, kSchedPad, 0, 0));
     // the block norm is summed by the last pass; it must see every amplitude
     {
         const PassDev &last = P.passes.back();

@@ -23,6 +23,7 @@
 //     pqqr coefficients whose inner orbital is occupied).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -35,9 +36,11 @@ enum OpType : int32_t {
     OP_DIAG = 0,   // diagonal phase (all pp + pqqp)
     OP_GA = 1,     // alpha excitation group (pq + gated pqqr)
     OP_GB = 2,     // beta excitation group
-    OP_AAAA = 3,   // pqrs, all alpha
-    OP_BBBB = 4,   // pqrs, all beta
-    OP_ABBA = 5    // pqrs, alpha p,s / beta q,r
+    OP_AAAA = 3,   // term classes of pqrs (the ops below carry them)
+    OP_BBBB = 4,
+    OP_ABBA = 5,
+    OP_FCC = 6,    // consecutive AAAA (COL pass) or BBBB (ROW pass) terms, one or more
+    OP_FAB = 7     // consecutive ABBA terms sharing (p, q, r), one or more
 };
 
 enum PassKind : int32_t { PASS_COL = 0, PASS_ROW = 1 };
@@ -47,23 +50,50 @@ enum PassFlags : int32_t {
     PF_EMIT_NORM = 2   // write per-workgroup FP64 partial sums of |a|^2
 };
 
+// Every op but DIAG carries a schedule: kThreadsPlan slots (one per thread of
+// the pass kernel) of lg <= kSchedMax entries, entry i of slot t at
+// sched[tab + i*kThreadsPlan + t].  A thread applies its entries in order, each
+// to every line of the tile (FCC, GA, GB) or every column pair (FAB).
 struct OpDev {
     int32_t type;
-    int32_t index;   // group index (GA/GB) or term index (pqrs)
-    uint32_t tab;    // offset of this op's pair entries in the CA or CCAA table
-    uint32_t aux;    // GA/GB: staged-angle slot; ABBA: q | r << 8 (beta orbitals)
-    double cs, sn;   // pqrs: cos(c t), sin(c t)
+    int32_t index;   // FCC/FAB: first term (coefficients coef[index .. index+g)); GA/GB: offset of the pair angles
+    uint32_t tab;    // offset of the schedule
+    uint32_t aux;    // GA/GB: staged-angle slot; FAB: q | r << 8 (beta orbitals)
+    int32_t lg;      // schedule rows
+    int32_t ns;      // schedule slots (kThreadsPlan)
+    int32_t g;       // terms in the group
+    int32_t pad;
 };
 
-// A pair entry packs two ranks and the Jordan-Wigner parity into 32 bits:
+// Schedule entry: two 14-bit ranks, the term's slot in the group (3 bits) and
+// the Jordan-Wigner parity.  Ranks equal to kSchedPad mark padding.
+constexpr uint32_t kSchedRankBits = 14;
+constexpr uint32_t kSchedPad = (1u << kSchedRankBits) - 1u;
+constexpr int kMaxGroup = 8;
+constexpr int kSchedMax = 4;        // schedule rows per op (entries per thread)
+constexpr int kThreadsPlan = 512;   // threads per workgroup of the pass kernel
+constexpr int kFuseSupportCC = 7;   // max spin-orbitals touched by a fused AAAA/BBBB group
+constexpr int kFuseSupportAB = 5;   // max alpha orbitals touched by a fused ABBA group
+inline uint32_t pack_sched(uint32_t r1, uint32_t r2, uint32_t slot, uint32_t par) {
+    return r1 | (r2 << kSchedRankBits) | (slot << 28) | (par << 31);
+}
+
+// A pair-table entry packs two ranks and the Jordan-Wigner parity into 32 bits:
 // bits 0-14 rank of the first string, 15-29 rank of the second, 31 parity.
-// pairs of one term a workgroup handles per op: 512 threads x 3 pairs each
-constexpr uint32_t kMaxPairs = 1536;
 
 constexpr uint32_t kRankBits = 15;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1u;
 inline uint32_t pack_pair(uint32_t r1, uint32_t r2, uint32_t par) {
     return r1 | (r2 << kRankBits) | (par << 31);
+}
+
+// Rotations are stored as (1 - cos a, sin a).  Applying u' = u - (1-cos a) u
+// + ... keeps the small quantity 1 - cos a at full relative precision when the
+// state is complex64: rounding cos a (close to 1) to FP32 would scale every
+// amplitude a term touches by the same factor on every Trotter step.
+inline double versine(double a) {
+    const double h = std::sin(0.5 * a);
+    return 2.0 * h * h;
 }
 
 struct GroupDev {
@@ -104,15 +134,16 @@ struct Plan {
     std::vector<uint32_t> strings;      // [C] the colex-ordered strings of one spin
     std::vector<double> theta_a, theta_b, cross;   // diag: [C], [C], [n][C]
     std::vector<GroupDev> groups;
-    std::vector<double> grp_a;          // [G][ca_pairs][2] cos, sin of the pair angle part
-    std::vector<double> grp_b;          // [G][C][2] cos, sin of the other-string angle part
+    std::vector<double> grp_a;          // GA/GB ops: [op.index + i*ns + t][2] 1-cos, sin of the pair angle part
+    std::vector<double> grp_b;          // [G][C][2] 1-cos, sin of the other-string angle part
     std::vector<TermDev> terms;
     std::vector<OpDev> ops;
     std::vector<PassDev> passes;
     std::vector<ItemDev> items;
     std::vector<uint32_t> col_list;     // column ranks of COL items
-    std::vector<uint32_t> ca_table;     // [(x*n+y)*ca_pairs + k]: rank(f|x), rank(f|y), parity
-    std::vector<uint32_t> ccaa_table;   // [cc*ccaa_pairs + k]: rank(f|r|s), rank(f|p|q), parity
+    std::vector<uint32_t> ca_table;     // host only: [(x*n+y)*ca_pairs + k]: rank(f|x), rank(f|y), parity
+    std::vector<uint32_t> sched;        // op schedules: [op.tab + i*op.ns + slot]
+    std::vector<double> coef;           // [term][2]: 1-cos(c t), sin(c t) of every double excitation
     size_t lds_fixed = 4096;            // LDS bytes before the tile
 
     // work accounting per U
@@ -122,8 +153,9 @@ struct Plan {
     int max_items = 0;
 };
 
-// Builds the plan.  tile_bytes: LDS budget for the amplitude tile.  Throws
-// std::runtime_error with a message on unsupported sizes.
-Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes);
+// Builds the plan.  tile_bytes: LDS budget for the amplitude tile.  fuse: group
+// consecutive double excitations with small joint support into one op.
+// Throws std::runtime_error with a message on unsupported sizes.
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse = true);
 
 }  // namespace qpe

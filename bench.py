@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--m", type=int, default=None, help="override the ancilla count (development only)")
+    ap.add_argument("--engine", type=int, default=0, help="0: fused groups of terms, 1: one term per op")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2, help="Trotter steps timed on the CPU oracle")
@@ -45,10 +46,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_qpe(h, cfg, m, dtype, psi0, steps, warmup):
+def run_qpe(h, cfg, m, dtype, psi0, steps, warmup, engine=0):
     n, e = cfg["n"], cfg["e"]
     M = 1 << m
     sv = qpesim.StateVector(m, n, e, dtype)
+    sv.set_option(qpesim.OPT_ENGINE, engine)
     scale = 1.0 / math.sqrt(M)
     rec = []
     for it in range(warmup + steps):
@@ -106,7 +108,7 @@ def main():
     log(f"workload {a.config}: n={2*n} spin-orbitals N={2*e} m={m} C={C} D={D} terms={h.term_counts()} tau={tau:.6e}")
     log(f"plan c64: {info32}\nplan c128: {info64}")
 
-    rec32, P32, y32 = run_qpe(h, cfg, m, qpesim.COMPLEX64, psi0, a.steps, a.warmup)
+    rec32, P32, y32 = run_qpe(h, cfg, m, qpesim.COMPLEX64, psi0, a.steps, a.warmup, a.engine)
     ev_ms = float(np.mean([r[1]["evolve_ms"] for r in rec32]))
     q_ms = float(np.mean([r[1]["qift_ms"] for r in rec32]))
     pk_ms = float(np.mean([r[1]["pk_ms"] for r in rec32]))
@@ -147,7 +149,7 @@ def main():
                        "bytes_per_launch": pass_bytes / passes, "avg_launch_us": avg_launch_s * 1e6}
 
     if not a.no_fp64:
-        rec64, P64, y64 = run_qpe(h, cfg, m, qpesim.COMPLEX128, psi0, 1, 0)
+        rec64, P64, y64 = run_qpe(h, cfg, m, qpesim.COMPLEX128, psi0, 1, 0, a.engine)
         ev64 = rec64[0][1]["evolve_ms"]
         wall64 = rec64[0][0]
         out["qpe_wall_s_fp64"] = wall64

@@ -120,7 +120,8 @@ int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs);
 enum qpe_option {
     QPE_OPT_RENORMALIZE = 0,   /* 1: renormalise every block to block 0's norm (default: 1 for
                                   complex64, 0 for complex128) */
-    QPE_OPT_ENGINE = 1,        /* 0: fused tile engine (default); 1: one launch per term */
+    QPE_OPT_ENGINE = 1,        /* 0: tile passes with fused groups of terms (default);
+                                  1: tile passes with one term per op */
     QPE_OPT_MAX_TILE_BYTES = 2 /* LDS bytes a tile may use (0: device maximum) */
 };
 int qpe_state_set_option(qpe_state_t s, int option, int64_t value);
