@@ -46,18 +46,42 @@ struct KSum {
     __device__ double value() const { return s + c; }
 };
 
+// Amplitude update x + d.  complex128 rounds to nearest.  complex64 rounds
+// stochastically: d is dithered by a uniform fraction of one ulp of x (rb, a
+// signed random byte, selects it in steps of ulp/256), so the stored value is
+// unbiased.  Round to nearest would drop every update below half an ulp: the
+// large amplitudes of a state would never lose the (1 - cos) part of the small
+// rotations that feed the small ones, a norm and population bias that grows
+// step after step.
+template <typename R> __device__ inline R upd(R x, R d, int rb);
+template <> __device__ inline double upd<double>(double x, double d, int) { return x + d; }
+template <> __device__ inline float upd<float>(float x, float d, int rb) {
+    const int eb = __float_as_int(x) & 0x7f800000;
+    const float q = __int_as_float(max(eb - (31 << 23), 0));   // ulp(x) / 256, 0 for zero and tiny x
+    return x + fmaf((float)rb + 0.5f, q, d);
+}
+__device__ inline int rbyte(uint32_t r, int k) { return (int)__builtin_amdgcn_sbfe(r, 8 * k, 8); }
+
+// 32-bit mixer for the dither bits (counter based: deterministic for a given plan)
+__device__ inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
 // 2x2 symmetric rotation {c, i*mu; i*mu, c} on the pair (u, v), given as
 // h = 1 - c.  The whole update -h*u + i*mu*v is formed first and added to u in
-// one rounding: for the small angles of a Trotter step h*u is often below half
-// an ulp of u, and u - h*u rounded on its own would drop it every time, a bias
-// that inflates the norm of complex64 states step after step.
+// one rounding (upd).  r: 32 random bits (complex64 dither).
 template <typename V, typename R>
-__device__ inline void rotate(V &u, V &v, R h, R mu) {
+__device__ inline void rotate(V &u, V &v, R h, R mu, uint32_t r) {
     V nu, nv;
-    nu.x = u.x + (-h * u.x - mu * v.y);
-    nu.y = u.y + (-h * u.y + mu * v.x);
-    nv.x = v.x + (-h * v.x - mu * u.y);
-    nv.y = v.y + (-h * v.y + mu * u.x);
+    nu.x = upd<R>(u.x, -h * u.x - mu * v.y, rbyte(r, 0));
+    nu.y = upd<R>(u.y, -h * u.y + mu * v.x, rbyte(r, 1));
+    nv.x = upd<R>(v.x, -h * v.x - mu * u.y, rbyte(r, 2));
+    nv.y = upd<R>(v.y, -h * v.y + mu * u.x, rbyte(r, 3));
     u = nu;
     v = nv;
 }
@@ -155,7 +179,11 @@ __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool vali
     for (int i = 0; i < kSchedMax; ++i) E.e[i] = base[(i < count ? i : 0) * stride];
 }
 
-__device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, int *pair_list, int *pair_count, int nl) {
+// Column pairs of (q, r) in the tile: lines l01 (r occupied, q empty) and its
+// partner l10, as LDS byte offsets; bit 0 of .x holds the Jordan-Wigner parity
+// of the pair (offsets are multiples of 8).
+__device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, int2 *pair_list, int *pair_count, int nl,
+                                       uint32_t line_bytes) {
     const int tid = threadIdx.x;
     const int q = (int)(qr & 0xff), r = (int)(qr >> 8);
     if (tid < 64) {
@@ -172,56 +200,64 @@ __device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, in
             }
         }
         uint64_t mask = __ballot(is01);
-        if (is01) pair_list[__popcll(mask & ((1ull << tid) - 1ull))] = tid | (partner << 8) | (parb << 16);
+        if (is01)
+            pair_list[__popcll(mask & ((1ull << tid) - 1ull))] =
+                make_int2((int)((uint32_t)tid * line_bytes) | parb, (int)((uint32_t)partner * line_bytes));
         if (tid == 0) *pair_count = __popcll(mask);
     }
 }
 
 // rotate (line[r1], line[r2]) on lines lane, lane + L, ... of the tile with one (h, mu)
 template <typename R, typename V>
-__device__ inline void rotate_lines(V *tile, uint32_t C, int nl, int lane, int L, uint32_t r1, uint32_t r2, R h,
-                                    R mu) {
-    for (int l0 = lane; l0 < nl; l0 += kChunk * L) {
+__device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1, uint32_t r2,
+                                    R h, R mu, uint32_t seed) {
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
+    const uint32_t step = (uint32_t)L * line_bytes;
+    uint32_t base = (uint32_t)lane * line_bytes;
+    for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (l0 + k * L < nl) {
-                u[k] = tile[(size_t)(l0 + k * L) * C + r1];
-                v[k] = tile[(size_t)(l0 + k * L) * C + r2];
+                u[k] = *reinterpret_cast<const V *>(tb + base + k * step + o1);
+                v[k] = *reinterpret_cast<const V *>(tb + base + k * step + o2);
             }
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (l0 + k * L < nl) {
-                rotate(u[k], v[k], h, mu);
-                tile[(size_t)(l0 + k * L) * C + r1] = u[k];
-                tile[(size_t)(l0 + k * L) * C + r2] = v[k];
+                rotate(u[k], v[k], h, mu, mix32(seed + (uint32_t)(l0 + k * L) * 0x9E3779B9u));
+                *reinterpret_cast<V *>(tb + base + k * step + o1) = u[k];
+                *reinterpret_cast<V *>(tb + base + k * step + o2) = v[k];
             }
     }
 }
 
 // rotate (l01[rs], l10[rp]) on column pairs lane, lane + L, ... of the pair list
 template <typename R, typename V>
-__device__ inline void rotate_pairs(V *tile, uint32_t C, const int *pair_list, int ncp, int lane, int L, uint32_t rs,
-                                    uint32_t rp, uint32_t par, R h, double sn) {
+__device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int lane, int L, uint32_t rs,
+                                    uint32_t rp, uint32_t par, R h, double sn, uint32_t seed) {
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t os = rs * (uint32_t)sizeof(V), op = rp * (uint32_t)sizeof(V);
     const R mpos = (R)sn, mneg = (R)(-sn);
     for (int x0 = lane; x0 < ncp; x0 += kChunk * L) {
-        int pl[kChunk];
+        int2 pl[kChunk];
 #pragma unroll
-        for (int k = 0; k < kChunk; ++k) pl[k] = (x0 + k * L < ncp) ? pair_list[x0 + k * L] : 0;
+        for (int k = 0; k < kChunk; ++k) pl[k] = pair_list[min(x0 + k * L, ncp - 1)];
         V u[kChunk], v[kChunk];
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (x0 + k * L < ncp) {
-                u[k] = tile[(size_t)(pl[k] & 0xff) * C + rs];
-                v[k] = tile[(size_t)((pl[k] >> 8) & 0xff) * C + rp];
+                u[k] = *reinterpret_cast<const V *>(tb + ((uint32_t)pl[k].x & ~7u) + os);
+                v[k] = *reinterpret_cast<const V *>(tb + (uint32_t)pl[k].y + op);
             }
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (x0 + k * L < ncp) {
-                const R mu = ((par ^ (uint32_t)(pl[k] >> 16)) & 1u) ? mpos : mneg;
-                rotate(u[k], v[k], h, mu);
-                tile[(size_t)(pl[k] & 0xff) * C + rs] = u[k];
-                tile[(size_t)((pl[k] >> 8) & 0xff) * C + rp] = v[k];
+                const R mu = ((par ^ (uint32_t)pl[k].x) & 1u) ? mpos : mneg;
+                rotate(u[k], v[k], h, mu, mix32(seed + (uint32_t)(x0 + k * L) * 0x9E3779B9u));
+                *reinterpret_cast<V *>(tb + ((uint32_t)pl[k].x & ~7u) + os) = u[k];
+                *reinterpret_cast<V *>(tb + (uint32_t)pl[k].y + op) = v[k];
             }
     }
 }
@@ -230,25 +266,29 @@ __device__ inline void rotate_pairs(V *tile, uint32_t C, const int *pair_list, i
 // plus the line's other-string part (gang[l]), combined by angle addition in
 // the (1 - cos, sin) form
 template <typename R, typename V>
-__device__ inline void rotate_group(V *tile, uint32_t C, int nl, int lane, int L, uint32_t r1, uint32_t r2,
-                                    uint32_t par, double ha, double sa, const double2 *ga) {
-    for (int l0 = lane; l0 < nl; l0 += kChunk * L) {
+__device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1, uint32_t r2,
+                                    uint32_t par, double ha, double sa, const double2 *ga, uint32_t seed) {
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
+    const uint32_t step = (uint32_t)L * line_bytes;
+    uint32_t base = (uint32_t)lane * line_bytes;
+    for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (l0 + k * L < nl) {
-                u[k] = tile[(size_t)(l0 + k * L) * C + r1];
-                v[k] = tile[(size_t)(l0 + k * L) * C + r2];
+                u[k] = *reinterpret_cast<const V *>(tb + base + k * step + o1);
+                v[k] = *reinterpret_cast<const V *>(tb + base + k * step + o2);
             }
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (l0 + k * L < nl) {
                 const double2 b = ga[l0 + k * L];
-                const double h = (ha + b.x) - ha * b.x + sa * b.y;
-                const double s = (sa + b.y) - sa * b.x - ha * b.y;
-                rotate(u[k], v[k], (R)h, (R)(par ? s : -s));
-                tile[(size_t)(l0 + k * L) * C + r1] = u[k];
-                tile[(size_t)(l0 + k * L) * C + r2] = v[k];
+                const double hh = (ha + b.x) - ha * b.x + sa * b.y;
+                const double ss = (sa + b.y) - sa * b.x - ha * b.y;
+                rotate(u[k], v[k], (R)hh, (R)(par ? ss : -ss), mix32(seed + (uint32_t)(l0 + k * L) * 0x9E3779B9u));
+                *reinterpret_cast<V *>(tb + base + k * step + o1) = u[k];
+                *reinterpret_cast<V *>(tb + base + k * step + o2) = v[k];
             }
     }
 }
@@ -256,15 +296,18 @@ __device__ inline void rotate_group(V *tile, uint32_t C, int nl, int lane, int L
 template <typename R>
 __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *coef, const EntSet &E,
                               typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
-                              const uint32_t *line_str, int *pair_list, int *pair_count, int nl, uint32_t C,
-                              uint32_t &cur_qr, int &ncp) {
+                              const uint32_t *line_str, int2 *pair_list, int *pair_count, int nl, uint32_t C,
+                              uint32_t &cur_qr, int &ncp, uint32_t op_seed) {
     using V = typename Vec<R>::type;
     const int tid = threadIdx.x;
     const int type = d.type;
+    const uint32_t line_bytes = C * (uint32_t)sizeof(V);
+    // dither bits: distinct per (step, pass, op, workgroup, thread, entry, line)
+    const uint32_t tseed = mix32(op_seed ^ ((blockIdx.x << 10) | (uint32_t)tid));
     if (type == OP_FAB) {
         if (d.aux != cur_qr) {
             cur_qr = d.aux;
-            build_pair_list(cur_qr, line_str, pair_list, pair_count, nl);
+            build_pair_list(cur_qr, line_str, pair_list, pair_count, nl, line_bytes);
             lds_barrier();
             ncp = *pair_count;
         }
@@ -275,7 +318,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
             const double2 cf = coef[s_slot(e)];
-            rotate_pairs<R>(tile, C, pair_list, ncp, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), (R)cf.x, cf.y);
+            rotate_pairs<R>(tile, pair_list, ncp, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), (R)cf.x, cf.y,
+                            tseed + (uint32_t)i * 0x632BE5ABu);
         }
     } else if (type == OP_FCC) {
         if (E.lane >= E.L) return;
@@ -285,7 +329,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
             const double2 cf = coef[s_slot(e)];
-            rotate_lines<R>(tile, C, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x, (R)(s_par(e) ? cf.y : -cf.y));
+            rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x,
+                            (R)(s_par(e) ? cf.y : -cf.y), tseed + (uint32_t)i * 0x632BE5ABu);
         }
     } else if (type == OP_GA || type == OP_GB) {
         if (E.lane >= E.L) return;
@@ -296,7 +341,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
             const double *gp = A.grp_a + 2ull * ((size_t)d.index + (size_t)i * d.ns + E.slot);
-            rotate_group<R>(tile, C, nl, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), gp[0], gp[1], ga);
+            rotate_group<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), gp[0], gp[1], ga,
+                            tseed + (uint32_t)i * 0x632BE5ABu);
         }
     } else if (type == OP_DIAG) {
         const uint32_t total = (uint32_t)nl * C;
@@ -315,8 +361,9 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             sincos(0.5 * th, &s2, &c2);
             V v = tile[idx], w;
             const R h = (R)(2.0 * s2 * s2), s = (R)(2.0 * s2 * c2);
-            w.x = v.x + (s * v.y - h * v.x);
-            w.y = v.y + (-s * v.x - h * v.y);
+            const uint32_t r = mix32(tseed + idx * 0x9E3779B9u);
+            w.x = upd<R>(v.x, s * v.y - h * v.x, rbyte(r, 0));
+            w.y = upd<R>(v.y, -s * v.x - h * v.y, rbyte(r, 1));
             tile[idx] = w;
         }
     }
@@ -328,8 +375,10 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
 // C * sizeof(V) is a multiple of 16, else the copy goes element by element.
 constexpr int kCopyUnroll = 8;
 
-// Every unrolled copy loads from a clamped (always valid) index so that the
-// staging array stays in registers; only the writes are guarded.
+// Unrolled copies are branch-free: an index past the end is clamped to the
+// last element, whose value such a thread then loads and writes again (the
+// same value the owning thread writes).  Guarded writes would let the compiler
+// sink each load next to its write and serialise the copy.
 template <typename V, typename U>
 __device__ inline void copy_cols_in(const V *src, V *tile, const uint32_t *line_rank, int nl, uint32_t C, double sc) {
     const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));   // units per line
@@ -338,17 +387,15 @@ __device__ inline void copy_cols_in(const V *src, V *tile, const uint32_t *line_
     U *t = reinterpret_cast<U *>(tile);
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
         U r[kCopyUnroll];
+        uint32_t ii[kCopyUnroll];
 #pragma unroll
         for (int k = 0; k < kCopyUnroll; ++k) {
-            const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
-            const uint32_t l = i / upl, a = i - l * upl;
+            ii[k] = min(i0 + (uint32_t)k * kThreads, total - 1);
+            const uint32_t l = ii[k] / upl, a = ii[k] - l * upl;
             r[k] = s[(uint64_t)line_rank[l] * upl + a];
         }
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
-            const uint32_t i = i0 + (uint32_t)k * kThreads;
-            if (i < total) t[i] = r[k];
-        }
+        for (int k = 0; k < kCopyUnroll; ++k) t[ii[k]] = r[k];
     }
     if (sc != 1.0) {
         __syncthreads();
@@ -370,22 +417,20 @@ __device__ inline void copy_rows_in(const V *src, V *tile, uint32_t r0, int nl, 
     const auto f = (decltype(V().x))sc;
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
         V r[kCopyUnroll];
+        uint32_t tb[kCopyUnroll];
 #pragma unroll
         for (int k = 0; k < kCopyUnroll; ++k) {
             const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
             const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
             r[k] = src[(uint64_t)b * C + r0 + l];
+            tb[k] = l * C + b;
         }
 #pragma unroll
         for (int k = 0; k < kCopyUnroll; ++k) {
-            const uint32_t i = i0 + (uint32_t)k * kThreads;
-            if (i < total) {
-                const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
-                V v = r[k];
-                v.x *= f;
-                v.y *= f;
-                tile[l * C + b] = v;
-            }
+            V v = r[k];
+            v.x *= f;
+            v.y *= f;
+            tile[tb[k]] = v;
         }
     }
 }
@@ -395,20 +440,16 @@ __device__ inline void copy_rows_out(V *dst, const V *tile, uint32_t r0, int nl,
     const uint32_t total = (uint32_t)nl * C;
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
         V r[kCopyUnroll];
+        uint64_t gi[kCopyUnroll];
 #pragma unroll
         for (int k = 0; k < kCopyUnroll; ++k) {
             const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
             const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
             r[k] = tile[l * C + b];
+            gi[k] = (uint64_t)b * C + r0 + l;
         }
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
-            const uint32_t i = i0 + (uint32_t)k * kThreads;
-            if (i < total) {
-                const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
-                dst[(uint64_t)b * C + r0 + l] = r[k];
-            }
-        }
+        for (int k = 0; k < kCopyUnroll; ++k) dst[gi[k]] = r[k];
     }
 }
 
@@ -420,16 +461,16 @@ __device__ inline void copy_cols_out(V *dst, const V *tile, const uint32_t *line
     const U *t = reinterpret_cast<const U *>(tile);
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
         U r[kCopyUnroll];
-#pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) r[k] = t[min(i0 + (uint32_t)k * kThreads, total - 1)];
+        uint64_t gi[kCopyUnroll];
 #pragma unroll
         for (int k = 0; k < kCopyUnroll; ++k) {
-            const uint32_t i = i0 + (uint32_t)k * kThreads;
-            if (i < total) {
-                const uint32_t l = i / upl, a = i - l * upl;
-                d[(uint64_t)line_rank[l] * upl + a] = r[k];
-            }
+            const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
+            const uint32_t l = i / upl, a = i - l * upl;
+            r[k] = t[i];
+            gi[k] = (uint64_t)line_rank[l] * upl + a;
         }
+#pragma unroll
+        for (int k = 0; k < kCopyUnroll; ++k) d[gi[k]] = r[k];
     }
 }
 
@@ -440,11 +481,12 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t *line_rank = reinterpret_cast<uint32_t *>(smem);            // [kMaxLines]
     uint32_t *line_str = line_rank + kMaxLines;                         // [kMaxLines]
-    int *pair_list = reinterpret_cast<int *>(line_str + kMaxLines);     // [kMaxLines]
-    int *pair_count = pair_list + kMaxLines;                            // [4]
+    int2 *pair_list = reinterpret_cast<int2 *>(line_str + kMaxLines);   // [kMaxLines]
+    int *pair_count = reinterpret_cast<int *>(pair_list + kMaxLines);   // [4]
     double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kWaves]
-    OpDev *ring = reinterpret_cast<OpDev *>(smem + 1024);              // [2][kBatch]
-    double2 *cring = reinterpret_cast<double2 *>(smem + 1536);          // [2][kBatch][kMaxGroup]
+    OpDev *ring = reinterpret_cast<OpDev *>(smem + 1280);              // [2][kBatch]
+    double2 *cring = reinterpret_cast<double2 *>(smem + 1792);          // [2][kBatch][kMaxGroup], ends at 3840
+    static_assert(4 * kMaxLines * 4 + 16 + 16 * kWaves <= 1280, "LDS header overlaps the descriptor ring");
     V *tile = reinterpret_cast<V *>(smem + 4096);
 
     const int tid = threadIdx.x;
@@ -502,7 +544,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
             {
                 const OpDev d = rb[i];
                 run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, pair_list, pair_count, nl, C,
-                          cur_qr, ncp);
+                          cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u));
             }
             if (o + 1 >= o1) break;
             // entries of op o+2 (the next batch's first op after the last pair)
@@ -512,7 +554,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
             {
                 const OpDev d = rb[i + 1];
                 run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, pair_list, pair_count,
-                          nl, C, cur_qr, ncp);
+                          nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u));
             }
         }
     }
