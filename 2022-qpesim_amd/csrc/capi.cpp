@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -364,32 +365,48 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
     });
 }
 
+// Everything before the timed evolution: argument checks, the plan (cached per
+// Hamiltonian version and options), the status flag, and block 0's norm.
+static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h) {
+    check_state(s);
+    check_ham(h);
+    require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+    require(h->h.num_orbitals() == s->n && h->h.num_occupied() == s->e, QPE_ERR_ARG,
+            "hamiltonian (num_orbitals, num_occupied) does not match the state");
+    materialize_scale(s);
+    if (!s->plan || s->plan_h != h || s->plan_version != h->version) {
+        size_t budget = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
+        if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
+        qpe::Plan p;
+        try {
+            p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0);
+        } catch (const std::runtime_error &e) {
+            throw ApiError(QPE_ERR_ARG, e.what());
+        }
+        std::unique_ptr<qpe::DevicePlan> dp(new qpe::DevicePlan());
+        dp->upload(p, s->dtype == QPE_COMPLEX128 ? 0 : 1, s->stream);
+        s->plan = std::move(dp);
+        s->plan_h = h;
+        s->plan_version = h->version;
+    }
+    qpe::hip_check(hipMemsetAsync(s->status, 0, sizeof(int), s->stream), "zero status");
+    set_ones(s->scale, s->M, s->stream);
+    qpe::block_norm(s->dtype == QPE_COMPLEX128 ? 0 : 1, s->D, s->amp, 0, s->norms, s->scale, s->norm_partials, kNormParts, s->stream);
+}
+
+static void finish_evolve(qpe_state_t s, float ms) {
+    s->t_evolve = ms;
+    s->passes = (int)s->plan->plan.passes.size();
+    s->scale_pending = s->renorm != 0;
+    s->pk_valid = false;
+    int status = 0;
+    qpe::hip_check(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost), "status copy");
+    require(status == 0, QPE_ERR_NUMERIC, "non-finite amplitudes during evolution");
+}
+
 int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h) {
     return guarded([&] {
-        check_state(s);
-        check_ham(h);
-        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
-        require(h->h.num_orbitals() == s->n && h->h.num_occupied() == s->e, QPE_ERR_ARG,
-                "hamiltonian (num_orbitals, num_occupied) does not match the state");
-        materialize_scale(s);
-        if (!s->plan || s->plan_h != h || s->plan_version != h->version) {
-            size_t budget = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
-            if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
-            qpe::Plan p;
-            try {
-                p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0);
-            } catch (const std::runtime_error &e) {
-                throw ApiError(QPE_ERR_ARG, e.what());
-            }
-            std::unique_ptr<qpe::DevicePlan> dp(new qpe::DevicePlan());
-            dp->upload(p, s->dtype == QPE_COMPLEX128 ? 0 : 1, s->stream);
-            s->plan = std::move(dp);
-            s->plan_h = h;
-            s->plan_version = h->version;
-        }
-        qpe::hip_check(hipMemsetAsync(s->status, 0, sizeof(int), s->stream), "zero status");
-        set_ones(s->scale, s->M, s->stream);
-        qpe::block_norm(s->dtype == QPE_COMPLEX128 ? 0 : 1, s->D, s->amp, 0, s->norms, s->scale, s->norm_partials, kNormParts, s->stream);
+        prepare_evolve(s, h);
         hipEvent_t e0, e1;
         qpe::hip_check(hipEventCreate(&e0), "event");
         qpe::hip_check(hipEventCreate(&e1), "event");
@@ -401,13 +418,137 @@ int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        s->t_evolve = ms;
-        s->passes = (int)s->plan->plan.passes.size();
-        s->scale_pending = s->renorm != 0;
-        s->pk_valid = false;
-        int status = 0;
-        qpe::hip_check(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost), "status copy");
-        require(status == 0, QPE_ERR_NUMERIC, "non-finite amplitudes during evolution");
+        finish_evolve(s, ms);
+    });
+}
+
+// ---- batched evolution --------------------------------------------------------
+
+namespace {
+// the last batch's device tables and graph, reused while the members, their
+// plans and the flags are unchanged
+struct BatchCache {
+    std::vector<const void *> key;
+    std::unique_ptr<qpe::BatchRun> run;
+};
+BatchCache g_batch;
+std::mutex g_batch_mu;
+}  // namespace
+
+int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, int flags,
+                          double *elapsed_ms) {
+    return guarded([&] {
+        require(states != nullptr && hams != nullptr && count >= 1, QPE_ERR_ARG, "empty batch");
+        require((flags & ~QPE_BATCH_EAGER) == 0, QPE_ERR_ARG, "unknown batch flags");
+        for (int i = 0; i < count; ++i) {
+            check_state(states[i]);
+            for (int k = 0; k < i; ++k) require(states[k] != states[i], QPE_ERR_ARG, "a state appears twice in the batch");
+            require(states[i]->dtype == states[0]->dtype, QPE_ERR_ARG, "batch members must share the amplitude type");
+            require(states[i]->M == states[0]->M, QPE_ERR_ARG, "batch members must share the ancilla count");
+        }
+        for (int i = 0; i < count; ++i) prepare_evolve(states[i], hams[i]);
+        for (int i = 0; i < count; ++i) qpe::hip_check(hipStreamSynchronize(states[i]->stream), "batch prepare sync");
+        std::lock_guard<std::mutex> lock(g_batch_mu);
+        std::vector<const void *> key;
+        for (int i = 0; i < count; ++i) {
+            key.push_back(states[i]);
+            key.push_back(states[i]->plan.get());
+            key.push_back(reinterpret_cast<const void *>((uintptr_t)states[i]->renorm));
+        }
+        if (!g_batch.run || g_batch.key != key) {
+            std::vector<qpe::BatchMember> mem;
+            for (int i = 0; i < count; ++i) {
+                qpe::BatchMember b;
+                b.dp = states[i]->plan.get();
+                b.state = states[i]->amp;
+                b.norms = states[i]->norms;
+                b.scale = states[i]->scale;
+                b.renorm = states[i]->renorm;
+                b.status = states[i]->status;
+                mem.push_back(b);
+            }
+            g_batch.run.reset(new qpe::BatchRun());
+            g_batch.key.clear();
+            g_batch.run->build(mem, states[0]->M, states[0]->stream);
+            g_batch.key = key;
+        }
+        hipStream_t st = states[0]->stream;
+        const bool graph = (flags & QPE_BATCH_EAGER) == 0;
+        if (graph) g_batch.run->instantiate();   // capture and instantiation stay outside the timed region
+        hipEvent_t e0, e1;
+        qpe::hip_check(hipEventCreate(&e0), "event");
+        qpe::hip_check(hipEventCreate(&e1), "event");
+        qpe::hip_check(hipEventRecord(e0, st), "event record");
+        g_batch.run->run(st, graph);
+        qpe::hip_check(hipEventRecord(e1, st), "event record");
+        qpe::hip_check(hipEventSynchronize(e1), "batch sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (elapsed_ms) *elapsed_ms = ms;
+        for (int i = 0; i < count; ++i) {
+            states[i]->launches = (int)g_batch.run->launches.size() + 1;
+            finish_evolve(states[i], ms);
+        }
+    });
+}
+
+// ---- autotuner ------------------------------------------------------------------
+
+int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *result) {
+    return guarded([&] {
+        check_state(s);
+        check_ham(h);
+        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+        require(share_cus >= 1, QPE_ERR_ARG, "share_cus must be positive");
+        const size_t full = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
+        const size_t line = (size_t)s->C * s->elem;
+        // candidate tile widths (lines per workgroup) from the widest that fits down to 1
+        std::vector<size_t> budgets;
+        for (size_t w = std::min<size_t>(full / line, 64); w >= 1; w /= 2) budgets.push_back(std::min(full, w * line + 64));
+        // a two-block scratch state of the same shape: one Trotter step per trial
+        qpe_state_t t = nullptr;
+        int rc = qpe_state_create(&t, 1, s->n, s->e, s->dtype);
+        if (rc != QPE_OK) throw ApiError(rc, "autotune scratch state: " + std::string(qpe_last_error()));
+        std::unique_ptr<qpe_state, int (*)(qpe_state_t)> guard(t, qpe_state_destroy);
+        const double one[2] = {1.0, 0.0};
+        double best = 1e300, best_us = 0.0;
+        int64_t best_budget = 0;
+        int best_engine = 0;
+        for (size_t b : budgets) {
+            for (int engine = 0; engine < 2; ++engine) {
+                t->max_tile_bytes = (int64_t)b;
+                t->engine = engine;
+                t->plan_version = ~0ull;
+                t->plan.reset();
+                double us = 0.0;
+                bool ok = true;
+                for (int rep = 0; rep < 3 && ok; ++rep) {   // first run builds the plan
+                    if (qpe_state_set(t, 0, 1, one) != QPE_OK || qpe_time_evolve_over_terms(t, h) != QPE_OK) ok = false;
+                    else if (rep > 0) us += t->t_evolve * 1e3 / 2.0;
+                }
+                if (!ok) continue;   // e.g. a budget too small for a mixed double excitation
+                int wgs = 0;
+                for (const auto &ps : t->plan->plan.passes) wgs = std::max(wgs, ps.item_count);
+                const double cost = us * std::max(1.0, (double)wgs / (double)share_cus);
+                if (cost < best) {
+                    best = cost;
+                    best_us = us;
+                    best_budget = (int64_t)b;
+                    best_engine = engine;
+                }
+            }
+        }
+        require(best < 1e300, QPE_ERR_ARG, "no tile configuration fits this instance");
+        s->max_tile_bytes = best_budget;
+        s->engine = best_engine;
+        s->plan_version = ~0ull;
+        if (result) {
+            result[0] = (double)best_budget;
+            result[1] = (double)best_engine;
+            result[2] = best_us;
+        }
     });
 }
 

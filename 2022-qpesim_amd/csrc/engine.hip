@@ -99,6 +99,8 @@ struct PassArgs {
     const double *coef;
     double *partials;
     const double *scale;
+    void *state;          // batched launches: the instance's register (column-major blocks)
+    uint64_t D;           // amplitudes per block
     int n, e;
     uint32_t C, ca_pairs, ccaa_pairs;
     int renorm;
@@ -302,8 +304,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
     const int tid = threadIdx.x;
     const int type = d.type;
     const uint32_t line_bytes = C * (uint32_t)sizeof(V);
-    // dither bits: distinct per (step, pass, op, workgroup, thread, entry, line)
-    const uint32_t tseed = mix32(op_seed ^ ((blockIdx.x << 10) | (uint32_t)tid));
+    // dither bits: distinct per (step, op, tile item, thread, entry, line); op_seed carries the item
+    const uint32_t tseed = mix32(op_seed ^ (uint32_t)tid);
     if (type == OP_FAB) {
         if (d.aux != cur_qr) {
             cur_qr = d.aux;
@@ -474,9 +476,10 @@ __device__ inline void copy_cols_out(V *dst, const V *tile, const uint32_t *line
     }
 }
 
+// One tile item of one pass: load the item's lines, run the pass's ops, write back.
 template <typename R>
-__global__ void __launch_bounds__(kThreads)
-pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
+__device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass, uint32_t item,
+                                          typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
     using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t *line_rank = reinterpret_cast<uint32_t *>(smem);            // [kMaxLines]
@@ -490,7 +493,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     V *tile = reinterpret_cast<V *>(smem + 4096);
 
     const int tid = threadIdx.x;
-    const ItemDev it = A.items[pass.item_begin + blockIdx.x];
+    const ItemDev it = A.items[pass.item_begin + item];
     const int nl = (int)it.count;
     const uint32_t C = A.C;
     const bool col = pass.kind == PASS_COL;
@@ -544,7 +547,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
             {
                 const OpDev d = rb[i];
                 run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, pair_list, pair_count, nl, C,
-                          cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u));
+                          cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
             }
             if (o + 1 >= o1) break;
             // entries of op o+2 (the next batch's first op after the last pair)
@@ -554,7 +557,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
             {
                 const OpDev d = rb[i + 1];
                 run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, pair_list, pair_count,
-                          nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u));
+                          nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
             }
         }
     }
@@ -583,7 +586,7 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
                 tot.add(red[2 * w]);
                 tot.add(red[2 * w + 1]);
             }
-            A.partials[blockIdx.x] = tot.value();
+            A.partials[item] = tot.value();
         }
     }
 
@@ -596,9 +599,36 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     }
 }
 
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
+    pass_body<R>(A, pass, blockIdx.x, prev, cur, j);
+}
+
+// Batched form: one launch runs one pass of several instances.  Workgroups
+// [work[w].block_begin, work[w+1].block_begin) are the items of work[w]; the
+// step index is read from device memory (*jp) so that one captured graph of
+// a step serves every step.
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+batch_pass_kernel(const PassArgs *args, const BatchWork *work, int nwork, const uint64_t *jp) {
+    using V = typename Vec<R>::type;
+    int lo = 0, hi = nwork - 1;   // last w with block_begin <= blockIdx.x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (work[mid].block_begin <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchWork &bw = work[lo];
+    const PassArgs A = args[bw.inst];
+    const uint64_t j = *jp;
+    V *base = reinterpret_cast<V *>(A.state);
+    pass_body<R>(A, bw.pass, blockIdx.x - bw.block_begin, base + (j - 1) * A.D, base + j * A.D, j);
+}
+
 // sum the per-workgroup partials of block j; set norms[j] and scale[j]
-__global__ void norm_finalize(const double *partials, int count, uint64_t j, double *norms, double *scale,
-                              int *status) {
+__device__ inline void finalize_body(const double *partials, int count, uint64_t j, double *norms, double *scale,
+                                     int *status) {
     __shared__ double red[2 * 16];
     KSum acc;
     for (int i = threadIdx.x; i < count; i += blockDim.x) acc.add(partials[i]);
@@ -624,6 +654,35 @@ __global__ void norm_finalize(const double *partials, int count, uint64_t j, dou
         if (j > 0 && v > 0.0 && isfinite(v)) f = sqrt(norms[0] / v);
         scale[j] = f;
         if (!isfinite(v) && status) *status = 1;
+    }
+}
+
+__global__ void norm_finalize(const double *partials, int count, uint64_t j, double *norms, double *scale,
+                              int *status) {
+    finalize_body(partials, count, j, norms, scale, status);
+}
+
+// resets the batched step counter (j = 1) and the finalize counter
+__global__ void batch_reset(uint64_t *jp, unsigned *done) {
+    if (threadIdx.x == 0) {
+        *jp = 1;
+        *done = 0u;
+    }
+}
+
+// batched form: workgroup i finalizes instance i; the last workgroup to finish
+// advances the step counter (*jp) for the next step's launches
+__global__ void batch_finalize(const FinalizeArgs *fa, int count, uint64_t *jp, unsigned *done) {
+    const FinalizeArgs f = fa[blockIdx.x];
+    const uint64_t j = *jp;
+    finalize_body(f.partials, f.count, j, f.norms, f.scale, f.status);
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(done, 1u) == (unsigned)count - 1u) {
+            *done = 0u;
+            *jp = j + 1;
+            __threadfence();
+        }
     }
 }
 
@@ -882,10 +941,7 @@ void DevicePlan::release() {
     coef = nullptr;
 }
 
-template <typename R>
-static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
-                     int *status_flag, hipStream_t s, int *launches) {
-    using V = typename Vec<R>::type;
+static PassArgs make_args(const DevicePlan &dp, void *state, double *scale, int renorm) {
     const Plan &p = dp.plan;
     PassArgs A;
     A.binom = dp.binom; A.strings = dp.strings;
@@ -894,15 +950,31 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norm
     A.ops = dp.ops; A.items = dp.items; A.col_list = dp.col_list;
     A.sched = dp.sched; A.coef = dp.coef;
     A.partials = dp.partials; A.scale = scale;
+    A.state = state;
+    A.D = (uint64_t)p.C * p.C;
     A.n = p.n; A.e = p.e; A.C = p.C; A.ca_pairs = p.ca_pairs; A.ccaa_pairs = p.ccaa_pairs;
     A.renorm = renorm;
-    const uint64_t D = (uint64_t)p.C * p.C;
+    return A;
+}
+
+static int norm_items_of(const Plan &p) {
+    int k = 0;
+    for (const auto &ps : p.passes)
+        if (ps.flags & PF_EMIT_NORM) k = ps.item_count;
+    return k;
+}
+
+template <typename R>
+static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
+                     int *status_flag, hipStream_t s, int *launches) {
+    using V = typename Vec<R>::type;
+    const Plan &p = dp.plan;
+    const PassArgs A = make_args(dp, state, scale, renorm);
+    const uint64_t D = A.D;
     V *base = reinterpret_cast<V *>(state);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
-    int norm_items = 0;
-    for (const auto &ps : p.passes)
-        if (ps.flags & PF_EMIT_NORM) norm_items = ps.item_count;
+    const int norm_items = norm_items_of(p);
     for (uint64_t j = 1; j < M; ++j) {
         V *prev = base + (j - 1) * D, *cur = base + j * D;
         for (const auto &ps : p.passes) {
@@ -915,6 +987,137 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norm
     if (launches) *launches = (int)p.passes.size() + 1;
 }
 
+// ---- batched evolution ---------------------------------------------------------
+
+void BatchRun::build(const std::vector<BatchMember> &members, uint64_t M_, hipStream_t s) {
+    release();
+    if (members.empty()) throw std::runtime_error("empty batch");
+    count = (int)members.size();
+    dtype = members[0].dp->dtype;
+    M = M_;
+    std::vector<PassArgs> args;
+    std::vector<FinalizeArgs> fin;
+    size_t maxp = 0;
+    lds_bytes = 0;
+    for (const auto &m : members) {
+        if (m.dp->dtype != dtype) throw std::runtime_error("batch members must share the amplitude type");
+        args.push_back(make_args(*m.dp, m.state, m.scale, m.renorm));
+        FinalizeArgs f;
+        f.partials = m.dp->partials;
+        f.count = norm_items_of(m.dp->plan);
+        f.pad = 0;
+        f.norms = m.norms;
+        f.scale = m.scale;
+        f.status = m.status;
+        fin.push_back(f);
+        maxp = std::max(maxp, m.dp->plan.passes.size());
+        lds_bytes = std::max(lds_bytes, m.dp->lds_bytes);
+    }
+    // launch k of a step: pass k of every member that has one, larger grids first
+    std::vector<BatchWork> work;
+    launches.clear();
+    for (size_t k = 0; k < maxp; ++k) {
+        std::vector<int> order;
+        for (int i = 0; i < count; ++i)
+            if (k < members[i].dp->plan.passes.size()) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return members[a].dp->plan.passes[k].item_count > members[b].dp->plan.passes[k].item_count;
+        });
+        Launch L;
+        L.work_off = work.size();
+        L.nwork = (int)order.size();
+        L.blocks = 0;
+        for (int i : order) {
+            BatchWork w;
+            w.inst = i;
+            w.block_begin = L.blocks;
+            w.pad0 = w.pad1 = 0;
+            w.pass = members[i].dp->plan.passes[k];
+            L.blocks += (uint32_t)w.pass.item_count;
+            work.push_back(w);
+        }
+        launches.push_back(L);
+    }
+    PassArgs *a_dev = nullptr;
+    to_device(&a_dev, args, s);
+    this->args = a_dev;
+    to_device(&this->work, work, s);
+    to_device(&this->fin, fin, s);
+    hip_check(hipMalloc(reinterpret_cast<void **>(&jstep), sizeof(uint64_t)), "hipMalloc(step counter)");
+    hip_check(hipMalloc(reinterpret_cast<void **>(&done), sizeof(unsigned)), "hipMalloc(done counter)");
+    hip_check(hipStreamSynchronize(s), "batch upload sync");
+}
+
+template <typename R>
+static void batch_step(const BatchRun &b, hipStream_t s) {
+    for (const auto &L : b.launches) {
+        if (L.blocks == 0) continue;
+        hipLaunchKernelGGL(batch_pass_kernel<R>, dim3(L.blocks), dim3(kThreads), b.lds_bytes, s,
+                           reinterpret_cast<const PassArgs *>(b.args), (const BatchWork *)(b.work + L.work_off),
+                           L.nwork, (const uint64_t *)b.jstep);
+    }
+    hipLaunchKernelGGL(batch_finalize, dim3(b.count), dim3(1024), 0, s, (const FinalizeArgs *)b.fin, b.count, b.jstep,
+                       b.done);
+}
+
+void BatchRun::run(hipStream_t s, bool use_graph) {
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, jstep, done);
+    if (dtype == 0)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    else
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (!use_graph) {
+        for (uint64_t j = 1; j < M; ++j) {
+            if (dtype == 0) batch_step<double>(*this, s);
+            else batch_step<float>(*this, s);
+        }
+        hip_check(hipGetLastError(), "batch launch");
+        return;
+    }
+    instantiate();
+    hip_check(hipGraphLaunch(graph, s), "graph launch");
+}
+
+void BatchRun::instantiate() {
+    if (graph && graph_steps == (int)(M - 1)) return;
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    if (dtype == 0)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    else
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    // capture every step of the chain; each step's kernels read the step from *jstep
+    hipStream_t cs;
+    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+    hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (uint64_t j = 1; j < M; ++j) {
+        if (dtype == 0) batch_step<double>(*this, cs);
+        else batch_step<float>(*this, cs);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e1 = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    hip_check(e1, "end capture");
+    hipError_t e2 = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(e2, "graph instantiate");
+    graph_steps = (int)(M - 1);
+}
+
+void BatchRun::release() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    graph_steps = 0;
+    void *ptrs[] = {args, work, fin, jstep, done};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    args = nullptr; work = nullptr; fin = nullptr; jstep = nullptr; done = nullptr;
+    launches.clear();
+}
 void evolve(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
             int *status_flag, hipStream_t s, int *launches) {
     if (dp.dtype == 0) evolve_t<double>(dp, state, M, norms, scale, renorm, status_flag, s, launches);

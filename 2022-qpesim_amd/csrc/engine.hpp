@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "plan.hpp"
 
@@ -32,6 +33,59 @@ struct DevicePlan {
     void upload(const Plan &p, int dtype, hipStream_t s);
     void release();
     ~DevicePlan() { release(); }
+};
+
+// Batched launches: one launch runs pass k of every member that has one.
+struct BatchWork {
+    int32_t inst;          // member index
+    uint32_t block_begin;  // first workgroup of this member's items in the launch
+    int32_t pad0, pad1;
+    PassDev pass;
+};
+struct FinalizeArgs {
+    const double *partials;
+    int count;
+    int pad;
+    double *norms, *scale;
+    int *status;
+};
+
+// One member of a batched evolution (all members share dtype and M).
+struct BatchMember {
+    const DevicePlan *dp;
+    void *state;
+    double *norms, *scale;
+    int renorm;
+    int *status;
+};
+
+// Device tables and the captured graph of a batched evolution.
+struct BatchRun {
+    int dtype = 0;
+    uint64_t M = 0;
+    size_t lds_bytes = 0;
+    void *args = nullptr;          // PassArgs[count]
+    BatchWork *work = nullptr;     // all launches' work lists, concatenated
+    FinalizeArgs *fin = nullptr;   // [count]
+    uint64_t *jstep = nullptr;     // device step counter
+    unsigned *done = nullptr;      // finalize completion counter
+    int count = 0;
+    struct Launch {
+        size_t work_off;
+        int nwork;
+        uint32_t blocks;
+    };
+    std::vector<Launch> launches;  // one step
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;           // steps captured in `graph`
+
+    void build(const std::vector<BatchMember> &members, uint64_t M, hipStream_t s);
+    // captures steps 1..M-1 into one graph (once)
+    void instantiate();
+    // runs steps 1..M-1; with use_graph the whole chain is one graph launch
+    void run(hipStream_t s, bool use_graph);
+    void release();
+    ~BatchRun() { release(); }
 };
 
 // One time_evolve_over_terms: blocks 1..M-1 of `state` (device, column-major

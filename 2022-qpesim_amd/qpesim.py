@@ -32,8 +32,9 @@ EXPORTS = [
     "qpe_hamiltonian_get_terms", "qpe_state_create", "qpe_state_destroy", "qpe_state_size", "qpe_state_set",
     "qpe_state_get", "qpe_state_scale_block", "qpe_time_evolve_over_terms", "qpe_qift",
     "qpe_identify_phase_amplitudes", "qpe_state_set_option", "qpe_state_timings", "qpe_state_block_norms",
-    "qpe_debug_pairs", "qpe_plan_info", "qpe_last_error",
+    "qpe_debug_pairs", "qpe_plan_info", "qpe_batch_time_evolve", "qpe_autotune", "qpe_last_error",
 ]
+BATCH_EAGER = 1
 
 _lib = None
 
@@ -71,6 +72,8 @@ def load():
         "qpe_debug_pairs": [ci, ci, ci, P(ctypes.c_int32), P(ctypes.c_uint32), P(ctypes.c_uint32),
                             P(ctypes.c_uint32), u64, P(u64)],
         "qpe_plan_info": [vp, ci, u64, P(dbl)],
+        "qpe_batch_time_evolve": [P(vp), P(vp), ci, ci, P(dbl)],
+        "qpe_autotune": [vp, vp, ci, P(dbl)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -215,6 +218,27 @@ class StateVector:
 def time_evolve_over_terms(sv, h):
     """time_evolve_over_terms(sv, h, m), cpp/time_evolution_over_terms.hpp:92-139."""
     _call("qpe_time_evolve_over_terms", sv.handle, h.handle)
+
+
+def batch_time_evolve(states, hams, eager=False):
+    """Batching form: evolve states[i] under hams[i] in one launch sequence (one
+    hipGraph launch unless eager).  Returns the device time in ms."""
+    n = len(states)
+    if n != len(hams) or n == 0:
+        raise ValueError("states and hams must be non-empty and of equal length")
+    sa = (ctypes.c_void_p * n)(*[sv.handle.value for sv in states])
+    ha = (ctypes.c_void_p * n)(*[h.handle.value for h in hams])
+    ms = ctypes.c_double(0.0)
+    _call("qpe_batch_time_evolve", sa, ha, n, BATCH_EAGER if eager else 0, ctypes.byref(ms))
+    return ms.value
+
+
+def autotune(sv, h, share_cus=256):
+    """Pick the tile width and term grouping of sv for h by timing one step of
+    each candidate; returns {tile_bytes, engine, us_per_step}."""
+    out = np.zeros(3, np.float64)
+    _call("qpe_autotune", sv.handle, h.handle, int(share_cus), _dptr(out))
+    return {"tile_bytes": int(out[0]), "engine": int(out[1]), "us_per_step": float(out[2])}
 
 
 def plan_info(h, dtype=COMPLEX128, tile_bytes=0):

@@ -107,6 +107,28 @@ int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor);
  * the reference's term order. */
 int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h);
 
+/* Batching form (no counterpart in the reference, whose main evolves one
+ * Hamiltonian): time_evolve_over_terms for `count` independent instances in
+ * one launch sequence.  states[i] is evolved under hams[i] exactly as
+ * qpe_time_evolve_over_terms would (same results, bit for bit); all states
+ * share the amplitude type and the ancilla count.  Launch k of a Trotter step
+ * runs pass k of every instance; the whole chain of steps is captured into one
+ * hipGraph and launched once (QPE_BATCH_EAGER: plain launches instead).
+ * elapsed_ms (may be NULL): device time of the evolution.  Each state's
+ * evolve timing is set to the batch time. */
+#define QPE_BATCH_EAGER 1
+int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, int flags,
+                          double *elapsed_ms);
+
+/* Autotuner (no counterpart in the reference): times one Trotter step of h on
+ * a two-block scratch state for each candidate tile width (lines per
+ * workgroup, which sets how many workgroups and hence CUs the instance spreads
+ * over) and term grouping (QPE_OPT_ENGINE 0/1), and sets the best on s.  The
+ * cost of a candidate is its step time, scaled up when its widest pass needs
+ * more workgroups than share_cus (the CUs the instance may take in a batch).
+ * result (may be NULL) receives {tile bytes, engine, microseconds per step}. */
+int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *result);
+
 /* StateVector::qift(), cpp/state_vector.hpp:115-179: in place, forward DFT
  * (e^{-2 pi i jk/M}) along the ancilla axis, natural order, divided by sqrt(M). */
 int qpe_qift(qpe_state_t s);
