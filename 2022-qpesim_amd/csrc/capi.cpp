@@ -74,6 +74,7 @@ struct qpe_state {
     int renorm = 0;
     int engine = 0;
     int64_t max_tile_bytes = 0;
+    int order = 1;   // Trotter order
     bool scale_pending = false;
     bool pk_valid = false;
     int pk_grid = 0;
@@ -355,6 +356,11 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
             s->engine = (int)value;
             s->plan_version = ~0ull;
             break;
+        case QPE_OPT_TROTTER_ORDER:
+            require(value == 1 || value == 2, QPE_ERR_ARG, "Trotter order must be 1 or 2");
+            s->order = (int)value;
+            s->plan_version = ~0ull;
+            break;
         case QPE_OPT_MAX_TILE_BYTES:
             require(value >= 0, QPE_ERR_ARG, "negative tile budget");
             s->max_tile_bytes = value;
@@ -379,7 +385,7 @@ static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h) {
         if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
         qpe::Plan p;
         try {
-            p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0);
+            p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0, s->order);
         } catch (const std::runtime_error &e) {
             throw ApiError(QPE_ERR_ARG, e.what());
         }

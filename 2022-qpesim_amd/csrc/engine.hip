@@ -114,6 +114,13 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Barrier between two ops: LDS only for tiles in LDS; a full workgroup barrier
+// (global memory ordered too) when the ops work on global memory.
+__device__ inline void op_barrier(bool global_tile) {
+    if (global_tile) __syncthreads();
+    else lds_barrier();
+}
+
 
 // Op pipeline.  Ops run in batches of kBatch.  Descriptors and term
 // coefficients of batch b+1 are staged in LDS rings while batch b runs, so no
@@ -182,10 +189,12 @@ __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool vali
 }
 
 // Column pairs of (q, r) in the tile: lines l01 (r occupied, q empty) and its
-// partner l10, as LDS byte offsets; bit 0 of .x holds the Jordan-Wigner parity
-// of the pair (offsets are multiples of 8).
-__device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, int2 *pair_list, int *pair_count, int nl,
-                                       uint32_t line_bytes) {
+// partner l10, as byte offsets line_base[l] * line_bytes from the tile base
+// (line_base: the line's position in LDS, or its column rank when the tile is
+// the block in global memory); bit 0 of .x holds the Jordan-Wigner parity of
+// the pair (offsets are multiples of 8).
+__device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, const uint32_t *line_base,
+                                       int2 *pair_list, int *pair_count, int nl, uint32_t line_bytes) {
     const int tid = threadIdx.x;
     const int q = (int)(qr & 0xff), r = (int)(qr >> 8);
     if (tid < 64) {
@@ -204,7 +213,7 @@ __device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, in
         uint64_t mask = __ballot(is01);
         if (is01)
             pair_list[__popcll(mask & ((1ull << tid) - 1ull))] =
-                make_int2((int)((uint32_t)tid * line_bytes) | parb, (int)((uint32_t)partner * line_bytes));
+                make_int2((int)(line_base[tid] * line_bytes) | parb, (int)(line_base[partner] * line_bytes));
         if (tid == 0) *pair_count = __popcll(mask);
     }
 }
@@ -298,8 +307,8 @@ __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int la
 template <typename R>
 __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *coef, const EntSet &E,
                               typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
-                              const uint32_t *line_str, int2 *pair_list, int *pair_count, int nl, uint32_t C,
-                              uint32_t &cur_qr, int &ncp, uint32_t op_seed) {
+                              const uint32_t *line_str, const uint32_t *line_base, int2 *pair_list, int *pair_count,
+                              int nl, uint32_t C, uint32_t &cur_qr, int &ncp, uint32_t op_seed) {
     using V = typename Vec<R>::type;
     const int tid = threadIdx.x;
     const int type = d.type;
@@ -309,7 +318,7 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
     if (type == OP_FAB) {
         if (d.aux != cur_qr) {
             cur_qr = d.aux;
-            build_pair_list(cur_qr, line_str, pair_list, pair_count, nl, line_bytes);
+            build_pair_list(cur_qr, line_str, line_base, pair_list, pair_count, nl, line_bytes);
             lds_barrier();
             ncp = *pair_count;
         }
@@ -489,20 +498,24 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kWaves]
     OpDev *ring = reinterpret_cast<OpDev *>(smem + 1280);              // [2][kBatch]
     double2 *cring = reinterpret_cast<double2 *>(smem + 1792);          // [2][kBatch][kMaxGroup], ends at 3840
+    static_assert(3840 + 4 * kMaxLines <= 4096, "LDS header overlaps the tile");
     static_assert(4 * kMaxLines * 4 + 16 + 16 * kWaves <= 1280, "LDS header overlaps the descriptor ring");
-    V *tile = reinterpret_cast<V *>(smem + 4096);
+    const bool gmode = pass.kind == PASS_COLG;   // lines stay in global memory
+    V *tile = gmode ? cur : reinterpret_cast<V *>(smem + 4096);
+    uint32_t *line_pos = reinterpret_cast<uint32_t *>(smem + 3840);     // [kMaxLines] LDS line index (0..63)
 
     const int tid = threadIdx.x;
     const ItemDev it = A.items[pass.item_begin + item];
     const int nl = (int)it.count;
     const uint32_t C = A.C;
-    const bool col = pass.kind == PASS_COL;
+    const bool col = pass.kind != PASS_ROW;   // COL and COLG lines are columns from the column list
     double2 *gang = reinterpret_cast<double2 *>(smem + 4096 + (size_t)pass.max_lines * C * sizeof(V));
     const int o0 = pass.op_begin, o1 = pass.op_end;
 
     if (tid < nl) {
         uint32_t r = col ? A.col_list[it.begin + tid] : it.begin + tid;
         line_rank[tid] = r;
+        line_pos[tid] = (uint32_t)tid;
         line_str[tid] = A.strings[r];
     }
     stage_batch(A, ring, cring, o0, o1, 0);
@@ -521,7 +534,9 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     const double sc = (from_prev && A.renorm) ? A.scale[j - 1] : 1.0;
     const uint32_t total = (uint32_t)nl * C;
     const bool vec16 = (C * sizeof(V)) % 16 == 0;
-    if (col) {
+    if (gmode) {
+        // in place in global memory: nothing to stage
+    } else if (col) {
         if (vec16) copy_cols_in<V, uint4>(src, tile, line_rank, nl, C, sc);
         else copy_cols_in<V, V>(src, tile, line_rank, nl, C, sc);
     } else {
@@ -543,21 +558,21 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
             const OpDev *rb = ring + (b & 1) * kBatch;
             const double2 *cb = cring + (size_t)((b & 1) * kBatch) * kMaxGroup;
             load_entries(A, rb[i + 1], o + 1 < o1, E1);
-            lds_barrier();
+            op_barrier(gmode);
             {
                 const OpDev d = rb[i];
-                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, pair_list, pair_count, nl, C,
-                          cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
+                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, gmode ? line_rank : line_pos,
+                          pair_list, pair_count, nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
             }
             if (o + 1 >= o1) break;
             // entries of op o+2 (the next batch's first op after the last pair)
             const OpDev &dn = (i + 2 < kBatch) ? rb[i + 2] : ring[((b + 1) & 1) * kBatch];
             load_entries(A, dn, o + 2 < o1, E0);
-            lds_barrier();
+            op_barrier(gmode);
             {
                 const OpDev d = rb[i + 1];
-                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, pair_list, pair_count,
-                          nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
+                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str,
+                          gmode ? line_rank : line_pos, pair_list, pair_count, nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
             }
         }
     }
@@ -591,7 +606,9 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     }
 
     // ---- store the tile -----------------------------------------------------
-    if (col) {
+    if (gmode) {
+        // the ops wrote global memory directly
+    } else if (col) {
         if (vec16) copy_cols_out<V, uint4>(cur, tile, line_rank, nl, C);
         else copy_cols_out<V, V>(cur, tile, line_rank, nl, C);
     } else {
@@ -924,7 +941,8 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     lds_bytes = 0;
     for (const auto &ps : p.passes) {
         if (ps.max_lines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
-        size_t b = 4096 + (size_t)ps.max_lines * p.C * p.elem_bytes + (size_t)ps.group_count * ps.max_lines * 16;
+        const size_t tile = ps.kind == PASS_COLG ? 0 : (size_t)ps.max_lines * p.C * p.elem_bytes;
+        size_t b = 4096 + tile + (size_t)ps.group_count * ps.max_lines * 16;
         lds_bytes = std::max(lds_bytes, b);
     }
     hip_check(hipStreamSynchronize(s), "plan upload sync");

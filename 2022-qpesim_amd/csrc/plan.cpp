@@ -34,7 +34,7 @@ uint32_t expand_outside(uint32_t y, const std::vector<int> &tpos_sorted) {
 
 }  // namespace
 
-Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse) {
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse, int order) {
     if (!h.indexed()) throw std::runtime_error("IndexTerms has not been called");
     Plan P;
     const int n = static_cast<int>(h.num_orbitals());
@@ -49,7 +49,10 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     P.elem_bytes = elem_bytes;
     P.tile_bytes = tile_bytes;
     const uint32_t C = P.C;
-    const double t = h.timestep();
+    if (order != 1 && order != 2) throw std::runtime_error("Trotter order must be 1 or 2");
+    P.order = order;
+    // second order: the step is the forward sequence at t/2 followed by the reversed sequence at t/2
+    const double t = order == 2 ? 0.5 * h.timestep() : h.timestep();
     const auto &seg = h.segmented();
 
     P.strings.resize(C);
@@ -507,7 +510,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         pd.pad0 = pd.pad1 = pd.pad2 = 0;
         for (const auto &it : items) {
             P.items.push_back(it);
-            P.tile_bytes_moved += 2ull * it.count * line_bytes;
+            if (kind != PASS_COLG) P.tile_bytes_moved += 2ull * it.count * line_bytes;
         }
         P.max_items = std::max(P.max_items, pd.item_count);
         P.passes.push_back(pd);
@@ -558,6 +561,11 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         for (int k = k0; k <= k1; ++k) m = std::max<uint64_t>(m, B(tsize, k));
         return m;
     };
+    // Mixed windows need two columns at least.  When the LDS tile holds fewer
+    // (complex128 with C above ~5000), their passes rotate in place in global
+    // memory (PASS_COLG) with windows of up to kMaxLinesPlan columns.
+    const bool gwin = max_orbit(2, std::max(1, e - (n - 2)), std::min(e, 1)) > (uint64_t)W;
+    const int WW = gwin ? kMaxLinesPlan : W;
     size_t ri = 0;
     while (ri < rots.size() && rots[ri].cls != OP_BBBB) {
         int p = rots[ri].p;
@@ -583,12 +591,12 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
                 if (nm != tmask) {
                     int ts = popcnt(nm);
                     int k0 = std::max(1, e - (n - ts)), k1 = std::min(e, ts - 1);
-                    if (max_orbit(ts, k0, k1) > (uint64_t)W) break;
+                    if (max_orbit(ts, k0, k1) > (uint64_t)WW) break;
                     tmask = nm;
                 }
                 ++aj;
             }
-            if (aj == ai) throw std::runtime_error("tile budget too small for a mixed double excitation");
+            if (aj == ai) throw std::runtime_error("window limit too small for a mixed double excitation");
             std::vector<int> tpos = positions(tmask);
             int ts = (int)tpos.size();
             int k0 = std::max(1, e - (n - ts)), k1 = std::min(e, ts - 1);
@@ -606,7 +614,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
                 uint32_t ny = (uint32_t)B(n - ts, e - k), npat = (uint32_t)B(ts, k);
                 for (uint32_t yi = 0; yi < ny; ++yi) {
                     uint32_t y = expand_outside(unrank_of(yi, e - k, n - ts, B.tab), tpos);
-                    if (cur.count + npat > (uint32_t)W) {
+                    if (cur.count + npat > (uint32_t)WW) {
                         items.push_back(cur);
                         cur = ItemDev{(uint32_t)P.col_list.size(), 0};
                     }
@@ -618,7 +626,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
                 }
             }
             if (cur.count) items.push_back(cur);
-            add_pass(PASS_COL, 0, ob, items, W, 0, 0);
+            add_pass(gwin ? PASS_COLG : PASS_COL, 0, ob, items, WW, 0, 0);
             ai = aj;
         }
     }
@@ -629,13 +637,41 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         push_groups(bbbb, false, W);
         if ((int)P.ops.size() > ob) add_pass(PASS_ROW, 0, ob, chunk_rows(W), W, 0, 0);
     }
+    if (order == 2) {
+        // mirrored half: passes and the ops in each pass in reverse order; a
+        // fused group's schedule rows reversed, so every orbit sees the group's
+        // terms backwards (rows of a single term or an excitation group commute)
+        const size_t nf = P.passes.size();
+        for (size_t k = nf; k-- > 0;) {
+            PassDev pd = P.passes[k];
+            pd.flags = 0;
+            const int ob = (int)P.ops.size();
+            for (int o = P.passes[k].op_end - 1; o >= P.passes[k].op_begin; --o) {
+                OpDev op = P.ops[o];
+                if ((op.type == OP_FCC || op.type == OP_FAB) && op.g > 1) {
+                    const uint32_t ntab = (uint32_t)P.sched.size();
+                    for (int i = op.lg - 1; i >= 0; --i)
+                        for (int q = 0; q < op.ns; ++q) P.sched.push_back(P.sched[op.tab + (size_t)i * op.ns + q]);
+                    op.tab = ntab;
+                }
+                P.ops.push_back(op);
+            }
+            pd.op_begin = ob;
+            pd.op_end = (int)P.ops.size();
+            for (int i = 0; i < pd.item_count; ++i)
+                if (pd.kind != PASS_COLG) P.tile_bytes_moved += 2ull * P.items[pd.item_begin + i].count * line_bytes;
+            P.passes.push_back(pd);
+        }
+        P.pair_updates *= 2;
+        P.diag_updates *= 2;
+    }
     if (P.sched.empty()) P.sched.push_back(pack_sched(kSchedPad, kSchedPad, 0, 0));
     // the block norm is summed by the last pass; it must see every amplitude
     {
         const PassDev &last = P.passes.back();
         uint64_t covered = 0;
         for (int i = 0; i < last.item_count; ++i) covered += P.items[last.item_begin + i].count;
-        if (covered != C) add_pass(PASS_COL, 0, (int)P.ops.size(), chunk_columns(W), W, 0, 0);
+        if (covered != C || last.kind == PASS_COLG) add_pass(PASS_COL, 0, (int)P.ops.size(), chunk_columns(W), W, 0, 0);
     }
     P.passes.back().flags |= PF_EMIT_NORM;
     return P;

@@ -43,7 +43,10 @@ enum OpType : int32_t {
     OP_FAB = 7     // consecutive ABBA terms sharing (p, q, r), one or more
 };
 
-enum PassKind : int32_t { PASS_COL = 0, PASS_ROW = 1 };
+// COLG: like COL, but the lines stay in global memory (windows of mixed double
+// excitations when the LDS tile cannot hold two columns); its ops are FAB only.
+enum PassKind : int32_t { PASS_COL = 0, PASS_ROW = 1, PASS_COLG = 2 };
+constexpr int kMaxLinesPlan = 64;   // lines per workgroup (line tables in LDS)
 
 enum PassFlags : int32_t {
     PF_FROM_PREV = 1,  // read block j-1 (scaled by its renormalisation factor), write block j
@@ -125,6 +128,7 @@ struct ItemDev {
 
 struct Plan {
     int n = 0, e = 0;
+    int order = 1;             // Trotter order (1: the reference's sequence; 2: symmetric)
     uint32_t C = 0;
     uint32_t ca_pairs = 0;     // C(n-2, e-1)
     uint32_t ccaa_pairs = 0;   // C(n-4, e-2)
@@ -155,7 +159,9 @@ struct Plan {
 
 // Builds the plan.  tile_bytes: LDS budget for the amplitude tile.  fuse: group
 // consecutive double excitations with small joint support into one op.
+// order: 1 reproduces the reference's first-order product; 2 builds the
+// symmetric second-order step (forward sequence at t/2, then reversed at t/2).
 // Throws std::runtime_error with a message on unsupported sizes.
-Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse = true);
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse = true, int order = 1);
 
 }  // namespace qpe

@@ -144,7 +144,11 @@ enum qpe_option {
                                   complex64, 0 for complex128) */
     QPE_OPT_ENGINE = 1,        /* 0: tile passes with fused groups of terms (default);
                                   1: tile passes with one term per op */
-    QPE_OPT_MAX_TILE_BYTES = 2 /* LDS bytes a tile may use (0: device maximum) */
+    QPE_OPT_MAX_TILE_BYTES = 2, /* LDS bytes a tile may use (0: device maximum) */
+    QPE_OPT_TROTTER_ORDER = 3   /* 1: the reference's first-order product (default);
+                                   2: symmetric second order, the first-order sequence at
+                                   t/2 followed by the same sequence reversed at t/2 (no
+                                   counterpart in the reference) */
 };
 int qpe_state_set_option(qpe_state_t s, int option, int64_t value);
 
