@@ -27,6 +27,11 @@ import qpesim  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per pass_kernel<float> launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+# each scaled by the same counter's reading for kernels of known size (FETCH x1.997, WRITE x1.0);
+# see profiles/pass_kernel_n32N8m2_pmc_traffic.md.  Keyed by (spin-orbitals, electrons, dtype, order).
+MEASURED_TRAFFIC = {(32, 8, "c64", 1): 43.63e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -232,7 +237,7 @@ def main():
     avg_launch_s = ev_ms * 1e-3 / passes
     achieved = pass_bytes / passes / avg_launch_s / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((2 * n, 2 * e, "c64", 1)),
                        "kernel": "pass_kernel<float> (term-rotation tile pass)",
                        "bytes_per_launch": pass_bytes / passes, "avg_launch_us": avg_launch_s * 1e6}
 
