@@ -75,6 +75,7 @@ struct qpe_state {
     int engine = 0;
     int64_t max_tile_bytes = 0;
     int order = 1;   // Trotter order
+    uint64_t evolved_to = 0;   // blocks < evolved_to hold U^j psi (chained qpe_time_evolve_blocks)
     bool scale_pending = false;
     bool pk_valid = false;
     int pk_grid = 0;
@@ -411,20 +412,40 @@ static void finish_evolve(qpe_state_t s, float ms) {
 }
 
 int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h) {
+    uint64_t M = 0;
+    const int rc = guarded([&] {
+        check_state(s);
+        M = s->M;
+    });
+    if (rc != QPE_OK) return rc;
+    return qpe_time_evolve_blocks(s, h, 1, M);
+}
+
+int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin, uint64_t j_end) {
     return guarded([&] {
-        prepare_evolve(s, h);
+        check_state(s);
+        require(j_begin >= 1 && j_begin <= j_end && j_end <= s->M, QPE_ERR_ARG, "block range outside 1..2^m");
+        if (j_begin == 1) {
+            prepare_evolve(s, h);
+        } else {
+            // continuing a chain: block j_begin-1 and its norm come from an earlier call with the same plan
+            check_ham(h);
+            require(s->plan && s->plan_h == h && s->plan_version == h->version && s->evolved_to == j_begin,
+                    QPE_ERR_STATE, "blocks before j_begin have not been evolved with this Hamiltonian");
+        }
         hipEvent_t e0, e1;
         qpe::hip_check(hipEventCreate(&e0), "event");
         qpe::hip_check(hipEventCreate(&e1), "event");
         qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
-        qpe::evolve(*s->plan, s->amp, s->M, s->norms, s->scale, s->renorm, s->status, s->stream, &s->launches);
+        qpe::evolve(*s->plan, s->amp, j_begin, j_end, s->norms, s->scale, s->renorm, s->status, s->stream, &s->launches);
         qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
         qpe::hip_check(hipEventSynchronize(e1), "evolve sync");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        finish_evolve(s, ms);
+        finish_evolve(s, j_begin == 1 ? ms : (float)s->t_evolve + ms);
+        s->evolved_to = j_end;
     });
 }
 
@@ -496,6 +517,7 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
         for (int i = 0; i < count; ++i) {
             states[i]->launches = (int)g_batch.run->launches.size() + 1;
             finish_evolve(states[i], ms);
+            states[i]->evolved_to = states[i]->M;
         }
     });
 }

@@ -983,8 +983,8 @@ static int norm_items_of(const Plan &p) {
 }
 
 template <typename R>
-static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
-                     int *status_flag, hipStream_t s, int *launches) {
+static void evolve_t(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms,
+                     double *scale, int renorm, int *status_flag, hipStream_t s, int *launches) {
     using V = typename Vec<R>::type;
     const Plan &p = dp.plan;
     const PassArgs A = make_args(dp, state, scale, renorm);
@@ -993,7 +993,7 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t M, double *norm
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     const int norm_items = norm_items_of(p);
-    for (uint64_t j = 1; j < M; ++j) {
+    for (uint64_t j = j_begin; j < j_end; ++j) {
         V *prev = base + (j - 1) * D, *cur = base + j * D;
         for (const auto &ps : p.passes) {
             hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.lds_bytes, s, A, ps, prev, cur, j);
@@ -1136,10 +1136,10 @@ void BatchRun::release() {
     args = nullptr; work = nullptr; fin = nullptr; jstep = nullptr; done = nullptr;
     launches.clear();
 }
-void evolve(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale, int renorm,
-            int *status_flag, hipStream_t s, int *launches) {
-    if (dp.dtype == 0) evolve_t<double>(dp, state, M, norms, scale, renorm, status_flag, s, launches);
-    else evolve_t<float>(dp, state, M, norms, scale, renorm, status_flag, s, launches);
+void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms, double *scale,
+            int renorm, int *status_flag, hipStream_t s, int *launches) {
+    if (dp.dtype == 0) evolve_t<double>(dp, state, j_begin, j_end, norms, scale, renorm, status_flag, s, launches);
+    else evolve_t<float>(dp, state, j_begin, j_end, norms, scale, renorm, status_flag, s, launches);
 }
 
 void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, double *scale, double *partials,

@@ -88,11 +88,11 @@ struct BatchRun {
     ~BatchRun() { release(); }
 };
 
-// One time_evolve_over_terms: blocks 1..M-1 of `state` (device, column-major
-// blocks of C*C amplitudes).  norms[M] and scale[M] are device arrays; norms[0]
-// and scale[0] must be set (block 0) before the call.  renorm selects whether
+// Blocks j_begin..j_end-1 of `state` (device, column-major blocks of C*C
+// amplitudes): block j = U block j-1.  norms[M] and scale[M] are device arrays;
+// norms and scale of blocks below j_begin must be set.  renorm selects whether
 // each block is rescaled to block 0's norm when it is read.
-void evolve(const DevicePlan &dp, void *state, uint64_t M, double *norms, double *scale,
+void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms, double *scale,
             int renorm, int *status_flag, hipStream_t s, int *launches_per_step);
 
 // squared norm of block j into norms[j] (and scale[j] = 1)

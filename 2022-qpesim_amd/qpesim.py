@@ -32,7 +32,7 @@ EXPORTS = [
     "qpe_hamiltonian_get_terms", "qpe_state_create", "qpe_state_destroy", "qpe_state_size", "qpe_state_set",
     "qpe_state_get", "qpe_state_scale_block", "qpe_time_evolve_over_terms", "qpe_qift",
     "qpe_identify_phase_amplitudes", "qpe_state_set_option", "qpe_state_timings", "qpe_state_block_norms",
-    "qpe_debug_pairs", "qpe_plan_info", "qpe_batch_time_evolve", "qpe_autotune", "qpe_last_error",
+    "qpe_debug_pairs", "qpe_plan_info", "qpe_batch_time_evolve", "qpe_autotune", "qpe_time_evolve_blocks", "qpe_last_error",
 ]
 BATCH_EAGER = 1
 
@@ -74,6 +74,7 @@ def load():
         "qpe_plan_info": [vp, ci, u64, P(dbl)],
         "qpe_batch_time_evolve": [P(vp), P(vp), ci, ci, P(dbl)],
         "qpe_autotune": [vp, vp, ci, P(dbl)],
+        "qpe_time_evolve_blocks": [vp, vp, u64, u64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -218,6 +219,11 @@ class StateVector:
 def time_evolve_over_terms(sv, h):
     """time_evolve_over_terms(sv, h, m), cpp/time_evolution_over_terms.hpp:92-139."""
     _call("qpe_time_evolve_over_terms", sv.handle, h.handle)
+
+
+def time_evolve_blocks(sv, h, j_begin, j_end):
+    """Blocks j_begin..j_end-1 of the chain (resumable pieces of time_evolve_over_terms)."""
+    _call("qpe_time_evolve_blocks", sv.handle, h.handle, int(j_begin), int(j_end))
 
 
 def batch_time_evolve(states, hams, eager=False):
@@ -383,7 +389,7 @@ CONFIGS = {
     "cfg0": dict(n=6, e=3, m=6, seed=0, hf=1.0),        # n=12 spin-orbitals, N=6
     "cfg1": dict(n=14, e=5, m=8, seed=1, hf=0.9),       # n=28, N=10
     "cfg2": dict(n=16, e=4, m=10, seed=2, hf=1.0),      # n=32, N=8
-    "cfg4": dict(n=17, e=5, m=7, seed=4, hf=0.95),      # n=34, N=10
+    "cfg4": dict(n=17, e=5, m=7, seed=4, hf=0.95, order=2),   # n=34, N=10, second-order Trotter
 }
 
 

@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--m", type=int, default=None, help="override the ancilla count (development only)")
     ap.add_argument("--engine", type=int, default=0, help="0: fused groups of terms, 1: one term per op")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="run only the complex128 QPE (reference P(k) for --ref-pk)")
+    ap.add_argument("--save-pk", default=None, help="write P(k) of the run to this .npy file")
+    ap.add_argument("--ref-pk", default=None, help="complex128 P(k) of the same instance from an earlier run (.npy)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2, help="Trotter steps timed on the CPU oracle")
     return ap.parse_args()
@@ -56,12 +59,19 @@ def run_qpe(h, cfg, m, dtype, psi0, steps, warmup, engine=0):
     M = 1 << m
     sv = qpesim.StateVector(m, n, e, dtype)
     sv.set_option(qpesim.OPT_ENGINE, engine)
+    sv.set_option(qpesim.OPT_TROTTER_ORDER, cfg.get("order", 1))
     scale = 1.0 / math.sqrt(M)
     rec = []
     for it in range(warmup + steps):
         sv.set(0, psi0 * scale)                 # inputs resident before the timed region
         t0 = time.perf_counter()
-        qpesim.time_evolve_over_terms(sv, h)
+        # the chain in 8 resumable pieces, each reporting its progress; device time accumulates
+        bounds = [1 + ((M - 1) * k) // 8 for k in range(9)]
+        for k in range(8):
+            if bounds[k + 1] > bounds[k]:
+                qpesim.time_evolve_blocks(sv, h, bounds[k], bounds[k + 1])
+                log(f"    evolved blocks {bounds[k]}..{bounds[k + 1] - 1} of {M - 1}: "
+                    f"{sv.timings()['evolve_ms']:.0f} ms device time so far")
         sv.qift()
         P = sv.identify_phase_amplitudes()
         wall = time.perf_counter() - t0
@@ -197,11 +207,23 @@ def main():
 
     info32 = qpesim.plan_info(h, qpesim.COMPLEX64)
     info64 = qpesim.plan_info(h, qpesim.COMPLEX128)
-    upd_per_step = info32["pair_updates"] + info32["diag_updates"]
+    order = cfg.get("order", 1)
+    # plan_info describes the first-order step; a second-order step applies every term twice (t/2 each)
+    upd_per_step = order * (info32["pair_updates"] + info32["diag_updates"])
     log(f"workload {a.config}: n={2*n} spin-orbitals N={2*e} m={m} C={C} D={D} terms={h.term_counts()} tau={tau:.6e}")
     log(f"plan c64: {info32}\nplan c128: {info64}")
 
+    if a.fp64_only:
+        rec64, P64, _ = run_qpe(h, cfg, m, qpesim.COMPLEX128, psi0, 1, 0, a.engine)
+        if a.save_pk:
+            np.save(a.save_pk, P64)
+        print(json.dumps({"fp64_only": True, "config": a.config, "m": m, "qpe_wall_s_fp64": rec64[0][0],
+                          "evolve_ms_fp64": rec64[0][1]["evolve_ms"], "sum_P": float(P64.sum()),
+                          "argmax_P": int(P64.argmax())}))
+        return
     rec32, P32, y32 = run_qpe(h, cfg, m, qpesim.COMPLEX64, psi0, a.steps, a.warmup, a.engine)
+    if a.save_pk:
+        np.save(a.save_pk, P32)
     ev_ms = float(np.mean([r[1]["evolve_ms"] for r in rec32]))
     q_ms = float(np.mean([r[1]["qift_ms"] for r in rec32]))
     pk_ms = float(np.mean([r[1]["pk_ms"] for r in rec32]))
@@ -223,8 +245,10 @@ def main():
         "dtype": "complex64 (fp32 state/rotations; fp64 angles, norms, P(k))",
         "data": "synthetic: seeded integrals (SplitMix64/Box-Muller), Hartree-Fock initial state",
         "config": {"workload": f"QPE {2*n} spin-orbitals, {2*e} electrons, m={m} ancillas "
-                               f"({M} x {C}^2 = {M*D} amplitudes, alpha x beta layout), first-order Trotter, seed {cfg['seed']}",
+                               f"({M} x {C}^2 = {M*D} amplitudes, alpha x beta layout), "
+                               f"{'second-order (symmetric)' if order == 2 else 'first-order'} Trotter, seed {cfg['seed']}",
                    "spin_orbitals": 2 * n, "electrons": 2 * e, "ancillas": m, "determinants_per_block": D,
+                   "trotter_order": order,
                    "parallelism": "single GPU"},
         "qpe_wall_s_fp32": wall32,
         "evolve_ms_fp32": ev_ms, "qift_ms_fp32": q_ms, "pk_ms_fp32": pk_ms,
@@ -232,8 +256,8 @@ def main():
     }
 
     # roofline of the dominant kernel (the term-rotation tile pass)
-    pass_bytes = info32["pass_bytes"] * (M - 1)
-    passes = info32["passes"] * (M - 1)
+    pass_bytes = order * info32["pass_bytes"] * (M - 1)
+    passes = order * info32["passes"] * (M - 1)
     avg_launch_s = ev_ms * 1e-3 / passes
     achieved = pass_bytes / passes / avg_launch_s / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -241,7 +265,11 @@ def main():
                        "kernel": "pass_kernel<float> (term-rotation tile pass)",
                        "bytes_per_launch": pass_bytes / passes, "avg_launch_us": avg_launch_s * 1e6}
 
-    if not a.no_fp64:
+    if a.ref_pk:
+        P64 = np.load(a.ref_pk)
+        out["max_abs_dP_fp32_vs_fp64"] = float(np.max(np.abs(P32 - P64)))
+        out["fp64_reference"] = f"P(k) of a complex128 run of the same instance ({a.ref_pk})"
+    elif not a.no_fp64:
         rec64, P64, y64 = run_qpe(h, cfg, m, qpesim.COMPLEX128, psi0, 1, 0, a.engine)
         ev64 = rec64[0][1]["evolve_ms"]
         wall64 = rec64[0][0]

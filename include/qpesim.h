@@ -107,6 +107,14 @@ int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor);
  * the reference's term order. */
 int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h);
 
+/* time_evolve_over_terms restricted to blocks j_begin .. j_end-1 (1 <= j_begin
+ * <= j_end <= 2^m): the reference's loop (TEOT:116-137) cut into resumable
+ * pieces, for callers that report progress or checkpoint between them.  A call
+ * with j_begin > 1 continues the chain of an earlier call with the same
+ * Hamiltonian that ended at j_begin (QPE_ERR_STATE otherwise).  Calls covering
+ * 1..2^m in order give exactly qpe_time_evolve_over_terms's result. */
+int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin, uint64_t j_end);
+
 /* Batching form (no counterpart in the reference, whose main evolves one
  * Hamiltonian): time_evolve_over_terms for `count` independent instances in
  * one launch sequence.  states[i] is evolved under hams[i] exactly as
