@@ -25,7 +25,7 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = kThreadsPlan;   // threads per pass workgroup (16 waves)
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxLines = 64;
 constexpr int kBinomWords = kBinomRows * kBinomCols;
@@ -130,7 +130,7 @@ __device__ inline void op_barrier(bool global_tile) {
 // previous op runs, into two alternating register sets.
 constexpr int kBatch = 8;
 constexpr int kDescWords = sizeof(OpDev) / 4;   // 8
-constexpr int kChunk = 8;                       // lines (column pairs) whose LDS loads are issued together
+constexpr int kChunk = 4;                       // lines (column pairs) whose LDS loads are issued together
 static_assert(kThreads == kThreadsPlan, "schedules are laid out for the pass kernel's thread count");
 
 // Schedule entries of one op for this thread.  The op's ns slots each get
@@ -385,13 +385,17 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
 // of C amplitudes; the line and its LDS image are 16-byte aligned when
 // C * sizeof(V) is a multiple of 16, else the copy goes element by element.
 constexpr int kCopyUnroll = 8;
+// 16-byte copy unit: a native vector type, loaded and stored as one value (a
+// struct type would be copied with memcpy, which can keep the staging array
+// out of registers)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Unrolled copies are branch-free: an index past the end is clamped to the
 // last element, whose value such a thread then loads and writes again (the
 // same value the owning thread writes).  Guarded writes would let the compiler
 // sink each load next to its write and serialise the copy.
 template <typename V, typename U>
-__device__ inline void copy_cols_in(const V *src, V *tile, const uint32_t *line_rank, int nl, uint32_t C, double sc) {
+__device__ __forceinline__ void copy_cols_in(const V *src, V *tile, const uint32_t *line_rank, int nl, uint32_t C, double sc) {
     const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));   // units per line
     const uint32_t total = (uint32_t)nl * upl;
     const U *s = reinterpret_cast<const U *>(src);
@@ -423,7 +427,7 @@ __device__ inline void copy_cols_in(const V *src, V *tile, const uint32_t *line_
 // ROW tiles: rows r0 .. r0+nl-1 (alpha strings) of all C columns, transposed
 // into lines of C amplitudes; global runs of nl consecutive amplitudes.
 template <typename V>
-__device__ inline void copy_rows_in(const V *src, V *tile, uint32_t r0, int nl, uint32_t C, double sc) {
+__device__ __forceinline__ void copy_rows_in(const V *src, V *tile, uint32_t r0, int nl, uint32_t C, double sc) {
     const uint32_t total = (uint32_t)nl * C;
     const auto f = (decltype(V().x))sc;
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
@@ -447,7 +451,7 @@ __device__ inline void copy_rows_in(const V *src, V *tile, uint32_t r0, int nl, 
 }
 
 template <typename V>
-__device__ inline void copy_rows_out(V *dst, const V *tile, uint32_t r0, int nl, uint32_t C) {
+__device__ __forceinline__ void copy_rows_out(V *dst, const V *tile, uint32_t r0, int nl, uint32_t C) {
     const uint32_t total = (uint32_t)nl * C;
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
         V r[kCopyUnroll];
@@ -465,7 +469,7 @@ __device__ inline void copy_rows_out(V *dst, const V *tile, uint32_t r0, int nl,
 }
 
 template <typename V, typename U>
-__device__ inline void copy_cols_out(V *dst, const V *tile, const uint32_t *line_rank, int nl, uint32_t C) {
+__device__ __forceinline__ void copy_cols_out(V *dst, const V *tile, const uint32_t *line_rank, int nl, uint32_t C) {
     const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));
     const uint32_t total = (uint32_t)nl * upl;
     U *d = reinterpret_cast<U *>(dst);
@@ -485,24 +489,77 @@ __device__ inline void copy_cols_out(V *dst, const V *tile, const uint32_t *line
     }
 }
 
+// The op loop of one tile item.  Ops run in batches of kBatch whose
+// descriptors and coefficients were staged in the LDS rings; each op's
+// schedule entries load one op ahead into the other register set.  GLOBAL: the
+// lines are columns of the block in global memory (gtile), else the LDS tile.
+template <typename R, bool GLOBAL>
+__device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, uint32_t item, uint64_t j,
+                                        typename Vec<R>::type *gtile, OpDev *ring, double2 *cring, EntSet &E0,
+                                        EntSet &E1, const double2 *gang, const uint32_t *line_rank,
+                                        const uint32_t *line_str, const uint32_t *line_base, int2 *pair_list,
+                                        int *pair_count, int nl, uint32_t C) {
+    using V = typename Vec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *tile = GLOBAL ? gtile : reinterpret_cast<V *>(smem + 4096);
+    const int o0 = pass.op_begin, o1 = pass.op_end;
+    uint32_t cur_qr = 0xffffffffu;
+    int ncp = 0;
+    const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
+    for (int b = 0; b < nbatch; ++b) {
+        __syncthreads();
+        if (b + 1 < nbatch) stage_batch(A, ring, cring, o0, o1, b + 1);
+        __syncthreads();
+        // two ops per iteration: the even op runs on E0 while E1 loads, the odd op the other way round
+        for (int i = 0; i < kBatch; i += 2) {
+            const int o = o0 + b * kBatch + i;
+            if (o >= o1) break;
+            const OpDev *rb = ring + (b & 1) * kBatch;
+            const double2 *cb = cring + (size_t)((b & 1) * kBatch) * kMaxGroup;
+            load_entries(A, rb[i + 1], o + 1 < o1, E1);
+            op_barrier(GLOBAL);
+            {
+                const OpDev d = rb[i];
+                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_list,
+                          pair_count, nl, C, cur_qr, ncp,
+                          mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
+            }
+            if (o + 1 >= o1) break;
+            // entries of op o+2 (the next batch's first op after the last pair)
+            const OpDev &dn = (i + 2 < kBatch) ? rb[i + 2] : ring[((b + 1) & 1) * kBatch];
+            load_entries(A, dn, o + 2 < o1, E0);
+            op_barrier(GLOBAL);
+            {
+                const OpDev d = rb[i + 1];
+                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_list,
+                          pair_count, nl, C, cur_qr, ncp,
+                          mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
+            }
+        }
+    }
+}
+
 // One tile item of one pass: load the item's lines, run the pass's ops, write back.
 template <typename R>
 __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass, uint32_t item,
                                           typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
     using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
+    // LDS header (bytes): line ranks [0,256), line strings [256,512), column pair list [512,1024)
+    // (reused for the norm reduction after the ops), pair count, LDS line index [1040,1296),
+    // descriptor ring [1296,1808), coefficient ring [1808,3856); the tile starts at 4096
     uint32_t *line_rank = reinterpret_cast<uint32_t *>(smem);            // [kMaxLines]
     uint32_t *line_str = line_rank + kMaxLines;                         // [kMaxLines]
     int2 *pair_list = reinterpret_cast<int2 *>(line_str + kMaxLines);   // [kMaxLines]
-    int *pair_count = reinterpret_cast<int *>(pair_list + kMaxLines);   // [4]
-    double *red = reinterpret_cast<double *>(pair_count + 4);           // [2 * kWaves]
-    OpDev *ring = reinterpret_cast<OpDev *>(smem + 1280);              // [2][kBatch]
-    double2 *cring = reinterpret_cast<double2 *>(smem + 1792);          // [2][kBatch][kMaxGroup], ends at 3840
-    static_assert(3840 + 4 * kMaxLines <= 4096, "LDS header overlaps the tile");
-    static_assert(4 * kMaxLines * 4 + 16 + 16 * kWaves <= 1280, "LDS header overlaps the descriptor ring");
+    double *red = reinterpret_cast<double *>(pair_list);                // [2 * kWaves], after the ops
+    int *pair_count = reinterpret_cast<int *>(smem + 1024);             // [4]
+    OpDev *ring = reinterpret_cast<OpDev *>(smem + 1296);              // [2][kBatch]
+    double2 *cring = reinterpret_cast<double2 *>(smem + 1808);          // [2][kBatch][kMaxGroup]
+    static_assert(16 * kWaves <= 8 * kMaxLines, "norm reduction does not fit the pair list");
+    static_assert(1808 + 2 * kBatch * kMaxGroup * 16 <= 4096, "LDS header overlaps the tile");
     const bool gmode = pass.kind == PASS_COLG;   // lines stay in global memory
-    V *tile = gmode ? cur : reinterpret_cast<V *>(smem + 4096);
-    uint32_t *line_pos = reinterpret_cast<uint32_t *>(smem + 3840);     // [kMaxLines] LDS line index (0..63)
+    V *tile = reinterpret_cast<V *>(smem + 4096);
+    uint32_t *line_pos = reinterpret_cast<uint32_t *>(smem + 1040);     // [kMaxLines] LDS line index (0..63)
 
     const int tid = threadIdx.x;
     const ItemDev it = A.items[pass.item_begin + item];
@@ -537,45 +594,21 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     if (gmode) {
         // in place in global memory: nothing to stage
     } else if (col) {
-        if (vec16) copy_cols_in<V, uint4>(src, tile, line_rank, nl, C, sc);
+        if (vec16) copy_cols_in<V, u32x4>(src, tile, line_rank, nl, C, sc);
         else copy_cols_in<V, V>(src, tile, line_rank, nl, C, sc);
     } else {
         copy_rows_in<V>(src, tile, it.begin, nl, C, sc);
     }
 
     // ---- ops ------------------------------------------------------------------
-    uint32_t cur_qr = 0xffffffffu;
-    int ncp = 0;
-    const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
-    for (int b = 0; b < nbatch; ++b) {
-        __syncthreads();
-        if (b + 1 < nbatch) stage_batch(A, ring, cring, o0, o1, b + 1);
-        __syncthreads();
-        // two ops per iteration: the even op runs on E0 while E1 loads, the odd op the other way round
-        for (int i = 0; i < kBatch; i += 2) {
-            const int o = o0 + b * kBatch + i;
-            if (o >= o1) break;
-            const OpDev *rb = ring + (b & 1) * kBatch;
-            const double2 *cb = cring + (size_t)((b & 1) * kBatch) * kMaxGroup;
-            load_entries(A, rb[i + 1], o + 1 < o1, E1);
-            op_barrier(gmode);
-            {
-                const OpDev d = rb[i];
-                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, gmode ? line_rank : line_pos,
-                          pair_list, pair_count, nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
-            }
-            if (o + 1 >= o1) break;
-            // entries of op o+2 (the next batch's first op after the last pair)
-            const OpDev &dn = (i + 2 < kBatch) ? rb[i + 2] : ring[((b + 1) & 1) * kBatch];
-            load_entries(A, dn, o + 2 < o1, E0);
-            op_barrier(gmode);
-            {
-                const OpDev d = rb[i + 1];
-                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str,
-                          gmode ? line_rank : line_pos, pair_list, pair_count, nl, C, cur_qr, ncp, mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
-            }
-        }
-    }
+    // two instantiations, so that each sees a single address space for the tile
+    // (LDS accesses stay ds_read/ds_write, global ones global_load/store)
+    if (gmode)
+        op_loop<R, true>(A, pass, item, j, cur, ring, cring, E0, E1, gang, line_rank, line_str, line_rank, pair_list,
+                         pair_count, nl, C);
+    else
+        op_loop<R, false>(A, pass, item, j, nullptr, ring, cring, E0, E1, gang, line_rank, line_str, line_pos,
+                          pair_list, pair_count, nl, C);
     __syncthreads();
 
     // ---- FP64 norm of the tile (last pass of a step) ----------------------
@@ -609,7 +642,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     if (gmode) {
         // the ops wrote global memory directly
     } else if (col) {
-        if (vec16) copy_cols_out<V, uint4>(cur, tile, line_rank, nl, C);
+        if (vec16) copy_cols_out<V, u32x4>(cur, tile, line_rank, nl, C);
         else copy_cols_out<V, V>(cur, tile, line_rank, nl, C);
     } else {
         copy_rows_out<V>(cur, tile, it.begin, nl, C);
@@ -622,13 +655,57 @@ pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<
     pass_body<R>(A, pass, blockIdx.x, prev, cur, j);
 }
 
+// Pointers loaded from a table carry no address space, so the compiler would
+// emit flat accesses (which also count against the LDS wait counter).  The
+// device reads the table through a mirror whose pointers are declared in the
+// global address space; converting them to the generic PassArgs keeps that
+// knowledge.
+#define QPE_GPTR(T) __attribute__((address_space(1))) T *
+struct PassArgsG {
+    QPE_GPTR(const uint32_t) binom;
+    QPE_GPTR(const uint32_t) strings;
+    QPE_GPTR(const double) theta_a;
+    QPE_GPTR(const double) theta_b;
+    QPE_GPTR(const double) cross;
+    QPE_GPTR(const GroupDev) groups;
+    QPE_GPTR(const double) grp_a;
+    QPE_GPTR(const double) grp_b;
+    QPE_GPTR(const OpDev) ops;
+    QPE_GPTR(const ItemDev) items;
+    QPE_GPTR(const uint32_t) col_list;
+    QPE_GPTR(const uint32_t) sched;
+    QPE_GPTR(const double) coef;
+    QPE_GPTR(double) partials;
+    QPE_GPTR(const double) scale;
+    QPE_GPTR(char) state;
+    uint64_t D;
+    int n, e;
+    uint32_t C, ca_pairs, ccaa_pairs;
+    int renorm;
+};
+static_assert(sizeof(PassArgsG) == sizeof(PassArgs), "PassArgsG mirrors PassArgs");
+
+__device__ inline PassArgs from_global_args(const PassArgsG &g) {
+    PassArgs A;
+    A.binom = (const uint32_t *)g.binom; A.strings = (const uint32_t *)g.strings;
+    A.theta_a = (const double *)g.theta_a; A.theta_b = (const double *)g.theta_b; A.cross = (const double *)g.cross;
+    A.groups = (const GroupDev *)g.groups; A.grp_a = (const double *)g.grp_a; A.grp_b = (const double *)g.grp_b;
+    A.ops = (const OpDev *)g.ops; A.items = (const ItemDev *)g.items; A.col_list = (const uint32_t *)g.col_list;
+    A.sched = (const uint32_t *)g.sched; A.coef = (const double *)g.coef; A.partials = (double *)g.partials;
+    A.scale = (const double *)g.scale; A.state = (void *)g.state;
+    A.D = g.D; A.n = g.n; A.e = g.e; A.C = g.C; A.ca_pairs = g.ca_pairs; A.ccaa_pairs = g.ccaa_pairs;
+    A.renorm = g.renorm;
+    return A;
+}
+
 // Batched form: one launch runs one pass of several instances.  Workgroups
 // [work[w].block_begin, work[w+1].block_begin) are the items of work[w]; the
 // step index is read from device memory (*jp) so that one captured graph of
 // a step serves every step.
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
-batch_pass_kernel(const PassArgs *args, const BatchWork *work, int nwork, const uint64_t *jp) {
+batch_pass_kernel(const PassArgs *__restrict__ args, const BatchWork *__restrict__ work, int nwork,
+                  const uint64_t *__restrict__ jp) {
     using V = typename Vec<R>::type;
     int lo = 0, hi = nwork - 1;   // last w with block_begin <= blockIdx.x
     while (lo < hi) {
@@ -637,7 +714,7 @@ batch_pass_kernel(const PassArgs *args, const BatchWork *work, int nwork, const 
         else hi = mid - 1;
     }
     const BatchWork &bw = work[lo];
-    const PassArgs A = args[bw.inst];
+    const PassArgs A = from_global_args(reinterpret_cast<const PassArgsG *>(args)[bw.inst]);
     const uint64_t j = *jp;
     V *base = reinterpret_cast<V *>(A.state);
     pass_body<R>(A, bw.pass, blockIdx.x - bw.block_begin, base + (j - 1) * A.D, base + j * A.D, j);

@@ -74,7 +74,7 @@ constexpr uint32_t kSchedRankBits = 14;
 constexpr uint32_t kSchedPad = (1u << kSchedRankBits) - 1u;
 constexpr int kMaxGroup = 8;
 constexpr int kSchedMax = 4;        // schedule rows per op (entries per thread)
-constexpr int kThreadsPlan = 512;   // threads per workgroup of the pass kernel
+constexpr int kThreadsPlan = 1024;  // threads per workgroup of the pass kernel
 constexpr int kFuseSupportCC = 7;   // max spin-orbitals touched by a fused AAAA/BBBB group
 constexpr int kFuseSupportAB = 5;   // max alpha orbitals touched by a fused ABBA group
 inline uint32_t pack_sched(uint32_t r1, uint32_t r2, uint32_t slot, uint32_t par) {
