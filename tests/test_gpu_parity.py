@@ -122,6 +122,34 @@ def test_fp64_parity_with_oracle(n, e, m, seed, hf):
     assert np.max(np.abs(norms - 1.0 / (1 << m))) < 1e-14  # every block keeps its norm
 
 
+@pytest.mark.slow
+@pytest.mark.parametrize("n,e,seed,hf", [(14, 5, 1, 0.9), (16, 4, 2, 1.0)])
+def test_fp64_parity_with_oracle_at_full_block_size(n, e, seed, hf):
+    """The block sizes of the n=28/N=10 and n=32/N=8 workloads (C = 2002, 1820),
+    three Trotter steps: amplitudes and P(k) against the oracle (about 20 s of
+    host time each)."""
+    m = 2
+    say(f"fp64 parity at full block size n={n} e={e}")
+    amps, y, P, norms, _ = gpu_run(n, e, m, seed, hf=hf)
+    ra, ry, rP = oracle_run(n, e, m, seed, hf=hf)
+    assert rel_l2(amps, ra) < AMP_TOL
+    assert rel_l2(y, ry) < AMP_TOL
+    assert np.max(np.abs(P - rP)) < PK_TOL
+
+
+def test_global_windows_at_full_block_size():
+    """n=34/N=10 (C = 6188): complex128 runs its mixed windows in global memory
+    (a column is 99 KB, the LDS tile holds one), complex64 in LDS tiles of three
+    columns.  Two independent code paths; one Trotter step must agree to the
+    complex64 precision."""
+    n, e, m, seed = 17, 5, 1, 4
+    say("global windows vs LDS windows at C=6188")
+    amps64, _, P64, _, _ = gpu_run(n, e, m, seed, hf=0.95)
+    amps32, _, P32, _, _ = gpu_run(n, e, m, seed, hf=0.95, dtype=qpesim.COMPLEX64)
+    assert rel_l2(amps32, amps64) < 1e-6
+    assert np.max(np.abs(P32 - P64)) < 1e-7
+
+
 @pytest.mark.parametrize("tile_cols", [2, 3, 5])
 def test_fp64_parity_small_tiles(tile_cols):
     """Small LDS budgets split the mixed double excitations into many windows."""
