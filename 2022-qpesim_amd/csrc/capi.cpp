@@ -4,13 +4,17 @@
 
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <memory>
+#include <sstream>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/qpesim.h"
+#include "broombridge.hpp"
 #include "combinadic.hpp"
 #include "engine.hpp"
 #include "hamiltonian.hpp"
@@ -58,6 +62,8 @@ uint64_t choose(int n, int k) {
 struct qpe_hamiltonian {
     qpe::Hamiltonian h;
     uint64_t version = 0;
+    double identity = 0.0;                    // GetIdentityTerm
+    std::vector<qpe::Determinant> initial;    // initial state of the last Broombridge ingest
 };
 
 struct qpe_state {
@@ -102,6 +108,30 @@ constexpr int kPkGridMax = 1024;
 
 void check_state(qpe_state_t s) { require(s != nullptr, QPE_ERR_ARG, "null state handle"); }
 void check_ham(qpe_hamiltonian_t h) { require(h != nullptr, QPE_ERR_ARG, "null hamiltonian handle"); }
+
+std::string read_text_file(const char *path) {
+    require(path != nullptr, QPE_ERR_ARG, "null path");
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw ApiError(QPE_ERR_ARG, std::string("cannot open ") + path);
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+qpe::BroombridgeData parse_broombridge(const qpe_hamiltonian *h, const std::string &text) {
+    try {
+        return qpe::read_broombridge(text, h->h.num_orbitals());
+    } catch (const qpe::FormatError &e) {
+        throw ApiError(QPE_ERR_ARG, std::string("Broombridge: ") + e.what());
+    }
+}
+
+void adopt_broombridge(qpe_hamiltonian *h, qpe::BroombridgeData &&d) {
+    qpe::apply_broombridge(d, h->h);
+    h->identity = d.identity;
+    h->initial = std::move(d.initial);
+    ++h->version;
+}
 
 void set_ones(double *dev, uint64_t count, hipStream_t st) {
     std::vector<double> ones(count, 1.0);
@@ -214,6 +244,52 @@ int qpe_hamiltonian_get_terms(qpe_hamiltonian_t h, int cls, int32_t *idx, double
                 if (idx) idx[4 * i + k] = v[i].idx[k];
             if (coeff) coeff[i] = v[i].coeff;
         }
+    });
+}
+
+int qpe_hamiltonian_ingest_broombridge(qpe_hamiltonian_t h, const char *path) {
+    return guarded([&] {
+        check_ham(h);
+        adopt_broombridge(h, parse_broombridge(h, read_text_file(path)));
+    });
+}
+
+int qpe_hamiltonian_ingest_broombridge_text(qpe_hamiltonian_t h, const char *text, uint64_t length) {
+    return guarded([&] {
+        check_ham(h);
+        require(text != nullptr || length == 0, QPE_ERR_ARG, "null text");
+        adopt_broombridge(h, parse_broombridge(h, std::string(text ? text : "", (size_t)length)));
+    });
+}
+
+int qpe_hamiltonian_initial_state(qpe_hamiltonian_t h, uint64_t *address, double *coeff, uint64_t cap,
+                                  uint64_t *count) {
+    return guarded([&] {
+        check_ham(h);
+        require(count != nullptr, QPE_ERR_ARG, "null count");
+        *count = h->initial.size();
+        if (!address && !coeff) return;
+        require(cap >= h->initial.size(), QPE_ERR_ARG, "capacity too small");
+        for (size_t i = 0; i < h->initial.size(); ++i) {
+            if (address) address[i] = h->initial[i].address;
+            if (coeff) coeff[i] = h->initial[i].coeff;
+        }
+    });
+}
+
+int qpe_hamiltonian_identity_term(qpe_hamiltonian_t h, double *identity) {
+    return guarded([&] {
+        check_ham(h);
+        require(identity != nullptr, QPE_ERR_ARG, "null output");
+        *identity = h->identity;
+    });
+}
+
+int qpe_hamiltonian_get_sizes(qpe_hamiltonian_t h, unsigned *num_orbitals, unsigned *num_occupied) {
+    return guarded([&] {
+        check_ham(h);
+        if (num_orbitals) *num_orbitals = h->h.num_orbitals();
+        if (num_occupied) *num_occupied = h->h.num_occupied();
     });
 }
 
@@ -344,6 +420,49 @@ int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor) {
         qpe::scale_block(s->dtype, s->amp, block, s->D, factor, s->stream);
         qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
         s->pk_valid = false;
+    });
+}
+
+int qpe_state_ingest_broombridge(qpe_state_t *out, qpe_hamiltonian_t h, const char *path, unsigned precision,
+                                 int dtype) {
+    return guarded([&] {
+        require(out != nullptr, QPE_ERR_ARG, "null output pointer");
+        *out = nullptr;
+        check_ham(h);
+        qpe::BroombridgeData d = parse_broombridge(h, read_text_file(path));
+        qpe_state_t s = nullptr;
+        auto check = [&](int rc) {
+            if (rc == QPE_OK) return;
+            std::string msg = g_last_error;
+            if (s) qpe_state_destroy(s);
+            throw ApiError(rc, msg);
+        };
+        check(qpe_state_create(&s, precision, d.num_orbitals, d.num_occupied, dtype));
+        // block 0: the determinants in file order, a repeated address keeping its last coefficient
+        std::map<uint64_t, double> amps;
+        for (const auto &det : d.initial) amps[det.address] = det.coeff;
+        for (const auto &kv : amps) {
+            const double a[2] = {kv.second, 0.0};
+            check(qpe_state_set(s, kv.first, 1, a));
+        }
+        adopt_broombridge(h, std::move(d));
+        *out = s;
+    });
+}
+
+int qpe_energy_estimate(const double *probs, unsigned precision, double identity, double timestep, double out[3]) {
+    return guarded([&] {
+        require(probs != nullptr && out != nullptr, QPE_ERR_ARG, "null pointer");
+        require(precision >= 1 && precision <= 30, QPE_ERR_ARG, "precision must be in [1, 30]");
+        require(std::isfinite(timestep) && timestep != 0.0, QPE_ERR_ARG, "timestep must be finite and non-zero");
+        const uint64_t M = 1ull << precision;
+        double total = 0.0;
+        for (uint64_t k = 0; k < M; ++k) total += (double)k * probs[k];
+        const double phase = total / (double)M;
+        const double two_pi = 2 * 2 * std::acos(0.0);
+        out[0] = -1.0 * two_pi * phase + identity;
+        out[1] = -1.0 * two_pi * phase / timestep + identity;
+        out[2] = two_pi / (timestep * (double)M);
     });
 }
 

@@ -33,6 +33,8 @@ EXPORTS = [
     "qpe_state_get", "qpe_state_scale_block", "qpe_time_evolve_over_terms", "qpe_qift",
     "qpe_identify_phase_amplitudes", "qpe_state_set_option", "qpe_state_timings", "qpe_state_block_norms",
     "qpe_debug_pairs", "qpe_plan_info", "qpe_batch_time_evolve", "qpe_autotune", "qpe_time_evolve_blocks", "qpe_last_error",
+    "qpe_hamiltonian_ingest_broombridge", "qpe_hamiltonian_ingest_broombridge_text", "qpe_hamiltonian_initial_state",
+    "qpe_hamiltonian_identity_term", "qpe_hamiltonian_get_sizes", "qpe_state_ingest_broombridge", "qpe_energy_estimate",
 ]
 BATCH_EAGER = 1
 
@@ -75,6 +77,13 @@ def load():
         "qpe_batch_time_evolve": [P(vp), P(vp), ci, ci, P(dbl)],
         "qpe_autotune": [vp, vp, ci, P(dbl)],
         "qpe_time_evolve_blocks": [vp, vp, u64, u64],
+        "qpe_hamiltonian_ingest_broombridge": [vp, ctypes.c_char_p],
+        "qpe_hamiltonian_ingest_broombridge_text": [vp, ctypes.c_char_p, u64],
+        "qpe_hamiltonian_initial_state": [vp, P(u64), P(dbl), u64, P(u64)],
+        "qpe_hamiltonian_identity_term": [vp, P(dbl)],
+        "qpe_hamiltonian_get_sizes": [vp, P(ctypes.c_uint), P(ctypes.c_uint)],
+        "qpe_state_ingest_broombridge": [P(vp), vp, ctypes.c_char_p, ctypes.c_uint, ci],
+        "qpe_energy_estimate": [P(dbl), ctypes.c_uint, dbl, dbl, P(dbl)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -143,6 +152,49 @@ class Hamiltonian:
     def sum_abs_coeff(self):
         return float(sum(np.abs(self.terms(c)[1]).sum() for c in range(5)))
 
+    def ingest_broombridge(self, path, precision=5, dtype=COMPLEX128):
+        """Hamiltonian::IngestBroombridge(path, precision), cpp/hamiltonian.cc:8-144:
+        adds the file's integrals and returns the StateVector of 2^precision blocks
+        whose block 0 holds the file's initial state."""
+        s = ctypes.c_void_p()
+        _call("qpe_state_ingest_broombridge", ctypes.byref(s), self._h, os.fsencode(path), int(precision), int(dtype))
+        n, e = self.sizes()
+        return StateVector._adopt(s, precision, n, e, dtype)
+
+    def load_broombridge(self, path=None, text=None):
+        """The Hamiltonian half of IngestBroombridge (no device state): integrals,
+        orbital and electron counts, identity term and initial determinants."""
+        if (path is None) == (text is None):
+            raise ValueError("give exactly one of path and text")
+        if path is not None:
+            _call("qpe_hamiltonian_ingest_broombridge", self._h, os.fsencode(path))
+        else:
+            b = text.encode() if isinstance(text, str) else bytes(text)
+            _call("qpe_hamiltonian_ingest_broombridge_text", self._h, b, len(b))
+
+    def initial_state(self):
+        """(addresses in block 0, signed coefficients) of the last ingest, file order."""
+        cnt = ctypes.c_uint64()
+        _call("qpe_hamiltonian_initial_state", self._h, None, None, 0, ctypes.byref(cnt))
+        n = int(cnt.value)
+        addr = np.zeros(max(n, 1), np.uint64)
+        co = np.zeros(max(n, 1), np.float64)
+        _call("qpe_hamiltonian_initial_state", self._h, addr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _dptr(co),
+              max(n, 1), ctypes.byref(cnt))
+        return addr[:n], co[:n]
+
+    def identity_term(self):
+        """Hamiltonian::GetIdentityTerm, cpp/hamiltonian.hpp:56."""
+        v = ctypes.c_double()
+        _call("qpe_hamiltonian_identity_term", self._h, ctypes.byref(v))
+        return v.value
+
+    def sizes(self):
+        """(num_orbitals(), num_occupied()), cpp/hamiltonian.hpp:68-69."""
+        n, e = ctypes.c_uint(), ctypes.c_uint()
+        _call("qpe_hamiltonian_get_sizes", self._h, ctypes.byref(n), ctypes.byref(e))
+        return int(n.value), int(e.value)
+
     @property
     def handle(self):
         return self._h
@@ -161,6 +213,19 @@ class StateVector:
         self.C = math.comb(self.n, self.e)
         self.D = self.C * self.C
         self.dtype = dtype
+
+    @classmethod
+    def _adopt(cls, handle, precision, num_orbitals, num_occupied, dtype):
+        sv = cls.__new__(cls)
+        sv._s = handle
+        sv.m = int(precision)
+        sv.M = 1 << sv.m
+        sv.n = int(num_orbitals)
+        sv.e = int(num_occupied)
+        sv.C = math.comb(sv.n, sv.e)
+        sv.D = sv.C * sv.C
+        sv.dtype = dtype
+        return sv
 
     def close(self):
         if getattr(self, "_s", None) and _lib is not None and self._s.value:
@@ -245,6 +310,42 @@ def autotune(sv, h, share_cus=256):
     out = np.zeros(3, np.float64)
     _call("qpe_autotune", sv.handle, h.handle, int(share_cus), _dptr(out))
     return {"tile_bytes": int(out[0]), "engine": int(out[1]), "us_per_step": float(out[2])}
+
+
+def energy_estimate(probs, precision, identity, timestep):
+    """Phase post-processing of cpp/main.cc:149-167: phase = sum_k k P(k) / 2^m;
+    `energy` is the reference's estimate -2 pi phase + identity (no division by
+    the timestep), `energy_per_time` = -2 pi phase / t + identity, `moe` the
+    reference's 2 pi / (t 2^m)."""
+    p = np.ascontiguousarray(probs, dtype=np.float64)
+    if p.size != 1 << int(precision):
+        raise ValueError("probs must hold 2^precision values")
+    out = np.zeros(3, np.float64)
+    _call("qpe_energy_estimate", _dptr(p), int(precision), float(identity), float(timestep), _dptr(out))
+    return {"phase": float(np.dot(np.arange(p.size, dtype=np.float64), p) / p.size), "energy": float(out[0]),
+            "energy_per_time": float(out[1]), "moe": float(out[2])}
+
+
+def run_main(path, precision, steps, dtype=COMPLEX128):
+    """The reference program's call sequence (cpp/main.cc:99-167) on one
+    Broombridge file: ingest, IndexTerms(HalfUp), SetTimestep(1/steps), block 0
+    scaled by 1/sqrt(2^m), time_evolve_over_terms, qift, IdentifyPhaseAmplitudes,
+    energy estimate."""
+    h = Hamiltonian()
+    sv = h.ingest_broombridge(path, precision, dtype)
+    try:
+        h.index_terms()
+        t = 1.0 / float(steps)
+        h.set_timestep(t)
+        sv.scale_block(0, 1.0 / math.sqrt(1 << int(precision)))
+        time_evolve_over_terms(sv, h)
+        sv.qift()
+        probs = sv.identify_phase_amplitudes()
+        est = energy_estimate(probs, precision, h.identity_term(), t)
+        est.update(probs=probs, timings=sv.timings(), num_orbitals=sv.n, num_occupied=sv.e)
+        return est
+    finally:
+        sv.close()
 
 
 def plan_info(h, dtype=COMPLEX128, tile_bytes=0):

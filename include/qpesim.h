@@ -72,6 +72,40 @@ int qpe_hamiltonian_index_terms(qpe_hamiltonian_t h);
 /* Hamiltonian::SetTimestep, cpp/hamiltonian.hpp:71 */
 int qpe_hamiltonian_set_timestep(qpe_hamiltonian_t h, double t);
 
+/* Hamiltonian::IngestBroombridge, cpp/hamiltonian.cc:8-144 (the Hamiltonian
+ * half; qpe_state_ingest_broombridge below is the whole call).  Reads a
+ * Broombridge v0.1/v0.2 file: one-body [p,q,v] (swapped to p >= q) and
+ * two-body Mulliken [i,l,j,k,v] -> AddTwoBodyTerm(i,j,k,l) integrals of
+ * integral_sets[0], 1-based; identity = coulomb_repulsion + energy_offset;
+ * num_orbitals = the largest integral index (running maximum with what h held);
+ * num_occupied and the initial determinants from
+ * initial_state_suggestions[0].state.superposition, each entry
+ * [coefficient, ladder operators..., vacuum] applied right to left with the
+ * Jordan-Wigner sign (-1)^(occupied modes below), modes alpha 0..n-1 then beta
+ * n..2n-1 (cpp/hamiltonian.cc:55-128, ParsePolishNotation :146-158).  The file
+ * is checked completely before h changes.  QPE_ERR_ARG (message names the
+ * line) for an unreadable file, malformed YAML or integrals, an operator string
+ * other than "(<k><a|b>)" / "(<k><a|b>)+", creation on an occupied or
+ * annihilation of an empty orbital (the reference asserts), unequal alpha and
+ * beta counts or determinants of different electron counts (the reference
+ * computes a meaningless address). */
+int qpe_hamiltonian_ingest_broombridge(qpe_hamiltonian_t h, const char *path);
+/* The same from a document held in memory (length bytes, no NUL needed). */
+int qpe_hamiltonian_ingest_broombridge_text(qpe_hamiltonian_t h, const char *text, uint64_t length);
+
+/* The initial state of the last ingest: *count determinants in file order,
+ * address[i] = C(n,e)*rank(alpha) + rank(beta) within block 0 and coeff[i]
+ * with its sign applied (cpp/hamiltonian.cc:117-128).  With address and coeff
+ * both NULL only *count is written; otherwise cap must be >= *count. */
+int qpe_hamiltonian_initial_state(qpe_hamiltonian_t h, uint64_t *address, double *coeff, uint64_t cap,
+                                  uint64_t *count);
+
+/* Hamiltonian::GetIdentityTerm, cpp/hamiltonian.hpp:56 (0 until an ingest) */
+int qpe_hamiltonian_identity_term(qpe_hamiltonian_t h, double *identity);
+
+/* Hamiltonian::num_orbitals() / num_occupied(), cpp/hamiltonian.hpp:68-69 */
+int qpe_hamiltonian_get_sizes(qpe_hamiltonian_t h, unsigned *num_orbitals, unsigned *num_occupied);
+
 /* sizes of the five lists PrintIndexedSegmentedHamiltonianTerms returns,
  * cpp/hamiltonian.cc:253-299 */
 int qpe_hamiltonian_term_counts(qpe_hamiltonian_t h, uint64_t counts[5]);
@@ -99,6 +133,14 @@ int qpe_state_get(qpe_state_t s, uint64_t offset, uint64_t count, double *amplit
 
 /* sv[j*C^2 + i] *= factor for all i: the implicit Hadamards of cpp/main.cc:111-115 */
 int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor);
+
+/* StateVector Hamiltonian::IngestBroombridge(path, precision),
+ * cpp/hamiltonian.cc:8-144: qpe_hamiltonian_ingest_broombridge, then a
+ * [2^precision][C][C] state of this dtype whose block 0 holds the initial
+ * determinants (a later duplicate address overwrites an earlier one).  On any
+ * error *out is NULL and h is unchanged. */
+int qpe_state_ingest_broombridge(qpe_state_t *out, qpe_hamiltonian_t h, const char *path, unsigned precision,
+                                 int dtype);
 
 /* ---------------- the hot path ---------------- */
 
@@ -144,6 +186,15 @@ int qpe_qift(qpe_state_t s);
 /* StateVector::IdentifyPhaseAmplitudes(), cpp/state_vector.hpp:341-372:
  * probs[k] = sum_d |y_k(d)|^2, k < 2^m (FP64, compensated sums). */
 int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs);
+
+/* Phase post-processing of cpp/main.cc:149-167 (host only): phase =
+ * (sum_k k*probs[k]) / 2^precision, summed in k order;
+ * out[0] = -2*pi*phase + identity, the reference's energy estimate (it does not
+ *          divide by the timestep);
+ * out[1] = -2*pi*phase / timestep + identity, the same phase read as
+ *          e^{-i E t}, i.e. divided by the timestep;
+ * out[2] = 2*pi / (timestep * 2^precision), the reference's margin of error. */
+int qpe_energy_estimate(const double *probs, unsigned precision, double identity, double timestep, double out[3]);
 
 /* ---------------- options, diagnostics ---------------- */
 
