@@ -81,6 +81,8 @@ struct qpe_state {
     int engine = 0;
     int64_t max_tile_bytes = 0;
     int order = 1;   // Trotter order
+    int term_order = 0;        // QPE_OPT_TERM_ORDER of the chain
+    int plan_term_order = -1;  // term order of the cached plan
     uint64_t evolved_to = 0;   // blocks < evolved_to hold U^j psi (chained qpe_time_evolve_blocks)
     bool scale_pending = false;
     bool pk_valid = false;
@@ -423,6 +425,36 @@ int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor) {
     });
 }
 
+static void apply_ancilla_gate(qpe_state_t s, const double *gate, int control, unsigned qubit) {
+    check_state(s);
+    require(gate != nullptr, QPE_ERR_ARG, "null gate");
+    for (int k = 0; k < 8; ++k) require(std::isfinite(gate[k]), QPE_ERR_ARG, "non-finite gate entry");
+    require(qubit < s->m, QPE_ERR_ARG, "qubit must be below the precision");
+    if (control >= 0) {
+        require((unsigned)control < s->m, QPE_ERR_ARG, "control must be below the precision");
+        require((unsigned)control != qubit, QPE_ERR_ARG, "control and target must differ");
+    }
+    materialize_scale(s);
+    // qubit q is bit m - q - 1 of the block index (big-endian, cpp/state_vector.hpp:56, 84-85)
+    const int tbit = (int)(s->m - qubit - 1), cbit = control >= 0 ? (int)(s->m - (unsigned)control - 1) : -1;
+    qpe::apply_gate(s->dtype, s->amp, s->D, (int)s->m, tbit, cbit, gate, s->stream);
+    qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
+    s->pk_valid = false;
+    s->evolved_to = 0;
+}
+
+int qpe_state_apply_gate(qpe_state_t s, const double *gate, unsigned qubit) {
+    return guarded([&] { apply_ancilla_gate(s, gate, -1, qubit); });
+}
+
+int qpe_state_apply_controlled_gate(qpe_state_t s, const double *gate, unsigned control, unsigned qubit) {
+    return guarded([&] {
+        check_state(s);
+        require(control < 64, QPE_ERR_ARG, "control must be below the precision");
+        apply_ancilla_gate(s, gate, (int)control, qubit);
+    });
+}
+
 int qpe_state_ingest_broombridge(qpe_state_t *out, qpe_hamiltonian_t h, const char *path, unsigned precision,
                                  int dtype) {
     return guarded([&] {
@@ -481,6 +513,11 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
             s->order = (int)value;
             s->plan_version = ~0ull;
             break;
+        case QPE_OPT_TERM_ORDER:
+            require(value == 0 || value == 1, QPE_ERR_ARG, "term order must be 0 (reordered) or 1 (applyUnitary)");
+            s->term_order = (int)value;
+            s->plan_version = ~0ull;
+            break;
         case QPE_OPT_MAX_TILE_BYTES:
             require(value >= 0, QPE_ERR_ARG, "negative tile budget");
             s->max_tile_bytes = value;
@@ -493,19 +530,19 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
 
 // Everything before the timed evolution: argument checks, the plan (cached per
 // Hamiltonian version and options), the status flag, and block 0's norm.
-static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h) {
+static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h, int term_order) {
     check_state(s);
     check_ham(h);
     require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
     require(h->h.num_orbitals() == s->n && h->h.num_occupied() == s->e, QPE_ERR_ARG,
             "hamiltonian (num_orbitals, num_occupied) does not match the state");
     materialize_scale(s);
-    if (!s->plan || s->plan_h != h || s->plan_version != h->version) {
+    if (!s->plan || s->plan_h != h || s->plan_version != h->version || s->plan_term_order != term_order) {
         size_t budget = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
         if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
         qpe::Plan p;
         try {
-            p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0, s->order);
+            p = qpe::build_plan(h->h, s->elem, budget, s->engine == 0, s->order, term_order);
         } catch (const std::runtime_error &e) {
             throw ApiError(QPE_ERR_ARG, e.what());
         }
@@ -514,6 +551,7 @@ static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h) {
         s->plan = std::move(dp);
         s->plan_h = h;
         s->plan_version = h->version;
+        s->plan_term_order = term_order;
     }
     qpe::hip_check(hipMemsetAsync(s->status, 0, sizeof(int), s->stream), "zero status");
     set_ones(s->scale, s->M, s->stream);
@@ -545,11 +583,12 @@ int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin,
         check_state(s);
         require(j_begin >= 1 && j_begin <= j_end && j_end <= s->M, QPE_ERR_ARG, "block range outside 1..2^m");
         if (j_begin == 1) {
-            prepare_evolve(s, h);
+            prepare_evolve(s, h, s->term_order);
         } else {
             // continuing a chain: block j_begin-1 and its norm come from an earlier call with the same plan
             check_ham(h);
-            require(s->plan && s->plan_h == h && s->plan_version == h->version && s->evolved_to == j_begin,
+            require(s->plan && s->plan_h == h && s->plan_version == h->version && s->evolved_to == j_begin &&
+                        s->plan_term_order == s->term_order,
                     QPE_ERR_STATE, "blocks before j_begin have not been evolved with this Hamiltonian");
         }
         hipEvent_t e0, e1;
@@ -565,6 +604,30 @@ int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin,
         (void)hipEventDestroy(e1);
         finish_evolve(s, j_begin == 1 ? ms : (float)s->t_evolve + ms);
         s->evolved_to = j_end;
+    });
+}
+
+int qpe_time_evolve_iqpe(qpe_state_t s, qpe_hamiltonian_t h, uint64_t num_of_applications) {
+    return guarded([&] {
+        check_state(s);
+        prepare_evolve(s, h, 1);
+        hipEvent_t e0, e1;
+        qpe::hip_check(hipEventCreate(&e0), "event");
+        qpe::hip_check(hipEventCreate(&e1), "event");
+        qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
+        qpe::evolve_in_place(*s->plan, s->amp, 1, num_of_applications, s->norms, s->scale, s->status, s->stream,
+                             &s->launches);
+        qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
+        qpe::hip_check(hipEventSynchronize(e1), "evolve sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const int renorm = s->renorm;
+        s->renorm = 0;   // no renormalisation in place: the scale factors stay 1
+        finish_evolve(s, ms);
+        s->renorm = renorm;
+        s->evolved_to = 0;   // block 1 no longer continues a chain
     });
 }
 
@@ -592,7 +655,7 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
             require(states[i]->dtype == states[0]->dtype, QPE_ERR_ARG, "batch members must share the amplitude type");
             require(states[i]->M == states[0]->M, QPE_ERR_ARG, "batch members must share the ancilla count");
         }
-        for (int i = 0; i < count; ++i) prepare_evolve(states[i], hams[i]);
+        for (int i = 0; i < count; ++i) prepare_evolve(states[i], hams[i], states[i]->term_order);
         for (int i = 0; i < count; ++i) qpe::hip_check(hipStreamSynchronize(states[i]->stream), "batch prepare sync");
         std::lock_guard<std::mutex> lock(g_batch_mu);
         std::vector<const void *> key;
@@ -667,6 +730,8 @@ int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *resu
             for (int engine = 0; engine < 2; ++engine) {
                 t->max_tile_bytes = (int64_t)b;
                 t->engine = engine;
+                t->order = s->order;
+                t->term_order = s->term_order;
                 t->plan_version = ~0ull;
                 t->plan.reset();
                 double us = 0.0;

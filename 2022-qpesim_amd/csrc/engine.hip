@@ -244,6 +244,25 @@ __device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int la
     }
 }
 
+// rotate_lines on the lines whose string has a bit of gate (one pqqr term of
+// applyUnitary's order whose inner orbital lies in the other string)
+template <typename R, typename V>
+__device__ inline void rotate_lines_gated(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1,
+                                          uint32_t r2, R h, R mu, const uint32_t *line_str, uint32_t gate,
+                                          uint32_t seed) {
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
+    for (int l = lane; l < nl; l += L) {
+        if (gate && !(line_str[l] & gate)) continue;
+        V *pu = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o1);
+        V *pv = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o2);
+        V u = *pu, v = *pv;
+        rotate(u, v, h, mu, mix32(seed + (uint32_t)l * 0x9E3779B9u));
+        *pu = u;
+        *pv = v;
+    }
+}
+
 // rotate (l01[rs], l10[rp]) on column pairs lane, lane + L, ... of the pair list
 template <typename R, typename V>
 __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int lane, int L, uint32_t rs,
@@ -343,6 +362,17 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x,
                             (R)(s_par(e) ? cf.y : -cf.y), tseed + (uint32_t)i * 0x632BE5ABu);
         }
+    } else if (type == OP_PQQR) {
+        if (E.lane >= E.L) return;
+#pragma unroll
+        for (int i = 0; i < kSchedMax; ++i) {
+            if (i >= d.lg) break;
+            const uint32_t e = E.e[i];
+            if (s_rank1(e) == kSchedPad) continue;
+            const double2 cf = coef[s_slot(e)];
+            rotate_lines_gated<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x,
+                                  (R)(s_par(e) ? cf.y : -cf.y), line_str, d.aux, tseed + (uint32_t)i * 0x632BE5ABu);
+        }
     } else if (type == OP_GA || type == OP_GB) {
         if (E.lane >= E.L) return;
         const double2 *ga = gang + (size_t)d.aux * nl;
@@ -361,10 +391,12 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             uint32_t l = idx / C, a = idx - l * C;
             uint32_t b = line_rank[l];
             uint32_t sa = A.strings[a];
-            double th = A.theta_a[a] + A.theta_b[b];
+            const uint64_t set = (uint64_t)d.index;   // diagonal table set
+            double th = A.theta_a[set * C + a] + A.theta_b[set * C + b];
+            const double *cross = A.cross + set * (uint64_t)A.n * C;
             while (sa) {
                 int p = __builtin_ctz(sa);
-                th += A.cross[(uint64_t)p * C + b];
+                th += cross[(uint64_t)p * C + b];
                 sa &= sa - 1;
             }
             // e^{-i th} = 1 - h - i s with h = 2 sin^2(th/2), s = sin th
@@ -957,6 +989,40 @@ __global__ void scale_kernel(typename Vec<R>::type *blk, uint64_t D, double f) {
     }
 }
 
+// 2x2 gate on ancilla bit tbit of the block index (blocks j0 with the bit 0
+// and j1 = j0 | bit), every amplitude of the two blocks; cbit >= 0 restricts
+// it to the blocks whose bit cbit is 1.  Threads stride over the amplitudes of
+// a block (coalesced); gridDim.y strides over the block pairs.  FP64 arithmetic.
+__device__ inline uint64_t insert_zero_bit(uint64_t x, int b) {
+    return ((x >> b) << (b + 1)) | (x & ((1ull << b) - 1ull));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) gate_kernel(typename Vec<R>::type *amp, uint64_t D, uint64_t npairs, int tbit,
+                                                   int cbit, double2 g00, double2 g01, double2 g10, double2 g11) {
+    for (uint64_t pi = blockIdx.y; pi < npairs; pi += gridDim.y) {
+        uint64_t j0;
+        if (cbit < 0) {
+            j0 = insert_zero_bit(pi, tbit);
+        } else {
+            const int lo = tbit < cbit ? tbit : cbit, hi = tbit < cbit ? cbit : tbit;
+            j0 = insert_zero_bit(insert_zero_bit(pi, lo), hi) | (1ull << cbit);
+        }
+        typename Vec<R>::type *b0 = amp + j0 * D, *b1 = amp + (j0 | (1ull << tbit)) * D;
+        for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < D; i += (uint64_t)gridDim.x * 256ull) {
+            const auto u = b0[i], v = b1[i];
+            const double ur = u.x, ui = u.y, vr = v.x, vi = v.y;
+            typename Vec<R>::type nu, nv;
+            nu.x = (R)(g00.x * ur - g00.y * ui + g01.x * vr - g01.y * vi);
+            nu.y = (R)(g00.x * ui + g00.y * ur + g01.x * vi + g01.y * vr);
+            nv.x = (R)(g10.x * ur - g10.y * ui + g11.x * vr - g11.y * vi);
+            nv.y = (R)(g10.x * ui + g10.y * ur + g11.x * vi + g11.y * vr);
+            b0[i] = nu;
+            b1[i] = nv;
+        }
+    }
+}
+
 __global__ void debug_pairs_kernel(int kind, int n, int e, int a0, int a1, int a2, int a3, uint32_t *o1,
                                    uint32_t *o2, uint32_t *op, uint64_t count) {
     __shared__ uint32_t tab[kBinomWords];
@@ -1077,6 +1143,28 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t j_begin, uint64
         }
         hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, j, norms,
                            scale, status_flag);
+    }
+    hip_check(hipGetLastError(), "pass launch");
+    if (launches) *launches = (int)p.passes.size() + 1;
+}
+
+template <typename R>
+static void evolve_in_place_t(const DevicePlan &dp, void *state, uint64_t block, uint64_t count, double *norms,
+                              double *scale, int *status_flag, hipStream_t s, int *launches) {
+    using V = typename Vec<R>::type;
+    const Plan &p = dp.plan;
+    const PassArgs A = make_args(dp, state, scale, 0);
+    V *cur = reinterpret_cast<V *>(state) + block * A.D;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    const int norm_items = norm_items_of(p);
+    for (uint64_t k = 0; k < count; ++k) {
+        // the step index only seeds the rounding dither (no renormalisation reads scale[])
+        for (const auto &ps : p.passes)
+            hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.lds_bytes, s, A, ps, cur, cur,
+                               k + 1);
+        hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, block,
+                           norms, scale, status_flag);
     }
     hip_check(hipGetLastError(), "pass launch");
     if (launches) *launches = (int)p.passes.size() + 1;
@@ -1213,6 +1301,12 @@ void BatchRun::release() {
     args = nullptr; work = nullptr; fin = nullptr; jstep = nullptr; done = nullptr;
     launches.clear();
 }
+void evolve_in_place(const DevicePlan &dp, void *state, uint64_t block, uint64_t count, double *norms, double *scale,
+                     int *status_flag, hipStream_t s, int *launches) {
+    if (dp.dtype == 0) evolve_in_place_t<double>(dp, state, block, count, norms, scale, status_flag, s, launches);
+    else evolve_in_place_t<float>(dp, state, block, count, norms, scale, status_flag, s, launches);
+}
+
 void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms, double *scale,
             int renorm, int *status_flag, hipStream_t s, int *launches) {
     if (dp.dtype == 0) evolve_t<double>(dp, state, j_begin, j_end, norms, scale, renorm, status_flag, s, launches);
@@ -1297,6 +1391,21 @@ void scale_block(int dtype, void *state, uint64_t j, uint64_t D, double factor, 
         hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(D)), dim3(256), 0, s,
                            reinterpret_cast<float2 *>(state) + j * D, D, factor);
     hip_check(hipGetLastError(), "scale launch");
+}
+
+void apply_gate(int dtype, void *state, uint64_t D, int m, int tbit, int cbit, const double *g, hipStream_t s) {
+    const uint64_t npairs = (1ull << m) >> (cbit < 0 ? 1 : 2);
+    const unsigned gx = (unsigned)std::min<uint64_t>((D + 255) / 256, 1024);
+    const unsigned gy = (unsigned)std::min<uint64_t>(npairs, std::max<uint64_t>(1, 4096 / gx));
+    const double2 g00 = make_double2(g[0], g[1]), g01 = make_double2(g[2], g[3]);
+    const double2 g10 = make_double2(g[4], g[5]), g11 = make_double2(g[6], g[7]);
+    if (dtype == 0)
+        hipLaunchKernelGGL(gate_kernel<double>, dim3(gx, gy), dim3(256), 0, s, reinterpret_cast<double2 *>(state), D,
+                           npairs, tbit, cbit, g00, g01, g10, g11);
+    else
+        hipLaunchKernelGGL(gate_kernel<float>, dim3(gx, gy), dim3(256), 0, s, reinterpret_cast<float2 *>(state), D,
+                           npairs, tbit, cbit, g00, g01, g10, g11);
+    hip_check(hipGetLastError(), "gate launch");
 }
 
 void debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *o1, uint32_t *o2, uint32_t *op,

@@ -95,6 +95,12 @@ struct BatchRun {
 void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms, double *scale,
             int renorm, int *status_flag, hipStream_t s, int *launches_per_step);
 
+// time_evolve_iqpe (cpp/time_evolution_over_terms.hpp:141-155): the plan's
+// step applied `count` times in place to block `block`, without
+// renormalisation; norms[block] receives the final squared norm.
+void evolve_in_place(const DevicePlan &dp, void *state, uint64_t block, uint64_t count, double *norms, double *scale,
+                     int *status_flag, hipStream_t s, int *launches);
+
 // squared norm of block j into norms[j] (and scale[j] = 1)
 void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, double *scale,
                 double *partials, int nparts, hipStream_t s);
@@ -110,6 +116,10 @@ void scatter_in(int dtype, void *state, const double *staging, uint64_t offset, 
 void gather_out(int dtype, const void *state, double *staging, uint64_t offset, uint64_t count,
                 uint32_t C, hipStream_t s);
 void scale_block(int dtype, void *state, uint64_t j, uint64_t D, double factor, hipStream_t s);
+
+// 2x2 gate g (row-major, interleaved re/im) on bit tbit of the block index of a
+// [2^m][D] register; cbit >= 0: only on blocks whose bit cbit is 1
+void apply_gate(int dtype, void *state, uint64_t D, int m, int tbit, int cbit, const double *g, hipStream_t s);
 
 // device-computed determinant pairs of one term (for bit-exact index tests)
 void debug_pairs(int kind, int n, int e, const int32_t *args, uint32_t *idx01, uint32_t *idx10,

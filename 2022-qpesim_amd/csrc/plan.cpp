@@ -34,7 +34,7 @@ uint32_t expand_outside(uint32_t y, const std::vector<int> &tpos_sorted) {
 
 }  // namespace
 
-Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse, int order) {
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse, int order, int term_order) {
     if (!h.indexed()) throw std::runtime_error("IndexTerms has not been called");
     Plan P;
     const int n = static_cast<int>(h.num_orbitals());
@@ -50,7 +50,10 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     P.tile_bytes = tile_bytes;
     const uint32_t C = P.C;
     if (order != 1 && order != 2) throw std::runtime_error("Trotter order must be 1 or 2");
+    if (term_order != 0 && term_order != 1) throw std::runtime_error("term order must be 0 or 1");
     P.order = order;
+    P.term_order = term_order;
+    const bool original = term_order == 1;
     // second order: the step is the forward sequence at t/2 followed by the reversed sequence at t/2
     const double t = order == 2 ? 0.5 * h.timestep() : h.timestep();
     const auto &seg = h.segmented();
@@ -75,6 +78,9 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     auto local = [n](int g) { return g < n ? g : g - n; };
 
     // ---- diagonal: pp (TEOT:376-395) and pqqp (TEOT:399-424) -------------
+    // One table set per DIAG op: the reordered step applies pp and pqqp back to
+    // back (one set); applyUnitary (TEOT:174-372) has the pq terms between them
+    // (set 0: pp, set 1: pqqp).
     std::vector<double> pp[2] = {std::vector<double>(n, 0.0), std::vector<double>(n, 0.0)};
     std::vector<double> vsame[2] = {std::vector<double>(n * n, 0.0), std::vector<double>(n * n, 0.0)};
     std::vector<double> vab(n * n, 0.0);
@@ -93,35 +99,50 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
             P.diag_updates += (uint64_t)B(n - 1, e - 1) * B(n - 1, e - 1);
         }
     }
-    P.theta_a.assign(C, 0.0);
-    P.theta_b.assign(C, 0.0);
-    P.cross.assign((size_t)n * C, 0.0);
-    for (uint32_t i = 0; i < C; ++i) {
-        uint32_t s = P.strings[i];
-        double ta = 0.0, tb = 0.0;
-        for (int p = 0; p < n; ++p) {
-            if (!(s & (1u << p))) continue;
-            ta += pp[0][p];
-            tb += pp[1][p];
-            for (int q = p + 1; q < n; ++q)
-                if (s & (1u << q)) {
-                    ta += vsame[0][p * n + q];
-                    tb += vsame[1][p * n + q];
+    auto add_diag_set = [&](bool with_pp, bool with_pqqp) {
+        const size_t a0 = P.theta_a.size(), c0 = P.cross.size();
+        P.theta_a.resize(a0 + C, 0.0);
+        P.theta_b.resize(a0 + C, 0.0);
+        P.cross.resize(c0 + (size_t)n * C, 0.0);
+        for (uint32_t i = 0; i < C; ++i) {
+            uint32_t s = P.strings[i];
+            double ta = 0.0, tb = 0.0;
+            for (int p = 0; p < n; ++p) {
+                if (!(s & (1u << p))) continue;
+                if (with_pp) {
+                    ta += pp[0][p];
+                    tb += pp[1][p];
                 }
+                if (with_pqqp)
+                    for (int q = p + 1; q < n; ++q)
+                        if (s & (1u << q)) {
+                            ta += vsame[0][p * n + q];
+                            tb += vsame[1][p * n + q];
+                        }
+            }
+            P.theta_a[a0 + i] = t * ta;
+            P.theta_b[a0 + i] = t * tb;
+            if (!with_pqqp) continue;
+            for (int p = 0; p < n; ++p) {
+                double w = 0.0;
+                for (int q = 0; q < n; ++q)
+                    if (s & (1u << q)) w += vab[p * n + q];
+                P.cross[c0 + (size_t)p * C + i] = t * w;
+            }
         }
-        P.theta_a[i] = t * ta;
-        P.theta_b[i] = t * tb;
-        for (int p = 0; p < n; ++p) {
-            double w = 0.0;
-            for (int q = 0; q < n; ++q)
-                if (s & (1u << q)) w += vab[p * n + q];
-            P.cross[(size_t)p * C + i] = t * w;
-        }
+    };
+    if (original) {
+        add_diag_set(true, false);
+        add_diag_set(false, true);
+    } else {
+        add_diag_set(true, true);
     }
 
     // ---- excitation groups: each pq term with the pqqr terms it gates (TEOT:428-513)
+    // (applyUnitary applies every pqqr term on its own, after the pqqp terms: no gating)
     std::map<std::pair<int, int>, std::vector<const IndexedTerm *>> gated;  // (p_, r_) -> pqqr terms
-    for (const auto &T : seg[kPQQR]) gated[{T.idx[0], T.idx[3]}].push_back(&T);
+    if (!original)
+        for (const auto &T : seg[kPQQR]) gated[{T.idx[0], T.idx[3]}].push_back(&T);
     const uint32_t npa = P.ca_pairs;
     std::vector<double> grp_pair;   // [G][npa][2] 1-cos, sin of the pair angle part, pair order k
     for (const auto &T : seg[kPQ]) {
@@ -173,6 +194,42 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
             P.grp_b[base_b + 2 * i + 1] = std::sin(ang);
         }
         P.groups.push_back(g);
+    }
+
+    // ---- applyUnitary's pqqr terms, each its own rotation (TEOT:254-301) ----
+    // Term (p_, q_, q_, r_), p_ < r_ of one spin: the pairs (f|r_, f|p_) of that
+    // spin's strings, with inner orbital q_ occupied: in f when q_ has the same
+    // spin (the pairs are filtered), in the other string when not (a line gate).
+    struct Single { int spin; uint32_t gate; int index; std::vector<uint32_t> entries; };
+    std::vector<Single> singles;
+    if (original) {
+        for (const auto &T : seg[kPQQR]) {
+            const int gp = T.idx[0], gi = T.idx[1], gr = T.idx[3];
+            const int sp = spin_of(gp);
+            const bool same = spin_of(gi) == sp;
+            if (same && e < 2) continue;   // updatePQQR_AAAA/BBBB return early (TEOT:1077-1080)
+            Single sg;
+            sg.spin = sp;
+            sg.gate = same ? 0u : (1u << local(gi));
+            sg.index = (int)P.terms.size();
+            const int x = local(gr), y = local(gp);
+            for (uint32_t k = 0; k < npa; ++k) {
+                uint32_t r1, r2, par;
+                const uint32_t f = ca_pair(k, x, y, n, e, B.tab, &r1, &r2, &par);
+                if (same && !(f & (1u << local(gi)))) continue;
+                sg.entries.push_back(pack_sched(r1, r2, 0, par));
+            }
+            TermDev d;
+            d.p = x; d.q = local(gi); d.r = y; d.s = y;
+            const double a = T.coeff * t;
+            d.cs = std::cos(a);
+            d.sn = std::sin(a);
+            P.terms.push_back(d);
+            P.coef.push_back(versine(a));
+            P.coef.push_back(d.sn);
+            P.pair_updates += same ? (uint64_t)B(n - 3, e - 2) * C : (uint64_t)npa * B(n - 1, e - 1);
+            singles.push_back(std::move(sg));
+        }
     }
 
     // ---- pqrs terms (TEOT:519-560) --------------------------------------
@@ -417,6 +474,36 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
             longest = std::max(longest, o.size());
         }
     };
+    // Emits the ops of one term group from its orbits: the slot count with the
+    // lowest cost among those whose schedule fits kSchedMax rows (a single term
+    // may exceed it at any slot count: its rows are then split over ops).
+    auto emit_orbits = [&](const Orbits &best, const std::vector<uint32_t> &order, int lg512, int type, int index,
+                           int g, uint32_t aux, int units) {
+        int ns_best = NS;
+        double bc = split_cost(std::min(lg512, kSchedMax), NS, units) * ((lg512 + kSchedMax - 1) / kSchedMax);
+        size_t total = 0, longest = 0;
+        orbit_stats(best, total, longest);
+        for (int L = 2; L <= std::min(NS, units); ++L) {   // more lanes than units idle
+            const int cand = NS / L;
+            if ((size_t)cand * kSchedMax < total || longest > (size_t)kSchedMax) break;
+            const int lower = (int)std::max<size_t>(longest, (total + cand - 1) / cand);
+            if (split_cost(lower, cand, units) >= bc - 1e-9) continue;
+            const int lgc = lpt_assign(best, order, cand, nullptr);
+            if (lgc > kSchedMax) continue;
+            const double c = split_cost(lgc, cand, units);
+            if (c < bc - 1e-9) { bc = c; ns_best = cand; }
+        }
+        Slots slot = pack_slots(best, order, ns_best);
+        const int rows = slot_len(slot);
+        for (int i0 = 0; i0 < std::max(rows, 1); i0 += kSchedMax) {
+            OpDev o = blank(type);
+            o.index = index;
+            o.g = g;
+            o.aux = aux;
+            emit_rows(slot, i0, std::min(kSchedMax, rows - i0), o);
+            P.ops.push_back(o);
+        }
+    };
     // Greedy grouping of consecutive terms: a group grows while its joint
     // support stays within the limit, no orbit is longer than kSchedMax and
     // the entries fill at most 3/4 of the NS x kSchedMax schedule (so the
@@ -465,34 +552,19 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
                 order = lpt_order(best);
                 lg512 = lpt_assign(best, order, NS, nullptr);
             }
-            // slot count with the lowest cost among those whose schedule fits kSchedMax rows
-            // (a single term may exceed it at any slot count: its rows are then split over ops)
-            int ns_best = NS;
-            double bc = split_cost(std::min(lg512, kSchedMax), NS, units) * ((lg512 + kSchedMax - 1) / kSchedMax);
-            size_t total = 0, longest = 0;
-            orbit_stats(best, total, longest);
-            for (int L = 2; L <= std::min(NS, units); ++L) {   // more lanes than units idle
-                const int cand = NS / L;
-                if ((size_t)cand * kSchedMax < total || longest > (size_t)kSchedMax) break;
-                const int lower = (int)std::max<size_t>(longest, (total + cand - 1) / cand);
-                if (split_cost(lower, cand, units) >= bc - 1e-9) continue;
-                const int lgc = lpt_assign(best, order, cand, nullptr);
-                if (lgc > kSchedMax) continue;
-                const double c = split_cost(lgc, cand, units);
-                if (c < bc - 1e-9) { bc = c; ns_best = cand; }
-            }
-            Slots slot = pack_slots(best, order, ns_best);
-            const int rows = slot_len(slot);
-            for (int i0 = 0; i0 < std::max(rows, 1); i0 += kSchedMax) {
-                OpDev o = blank(ab ? OP_FAB : OP_FCC);
-                o.index = grp[0].index;
-                o.g = (int)grp.size();
-                o.aux = ab ? ((uint32_t)grp[0].q | ((uint32_t)grp[0].r << 8)) : 0u;
-                emit_rows(slot, i0, std::min(kSchedMax, rows - i0), o);
-                P.ops.push_back(o);
-            }
+            emit_orbits(best, order, lg512, ab ? OP_FAB : OP_FCC, grp[0].index, (int)grp.size(),
+                        ab ? ((uint32_t)grp[0].q | ((uint32_t)grp[0].r << 8)) : 0u, units);
             i = j;
         }
+    };
+    // one applyUnitary pqqr term: every entry its own orbit
+    auto push_single = [&](const Single &sg, int units) {
+        if (sg.entries.empty()) return;
+        Orbits ob;
+        ob.reserve(sg.entries.size());
+        for (uint32_t en : sg.entries) ob.push_back({en});
+        std::vector<uint32_t> order = lpt_order(ob);
+        emit_orbits(ob, order, lpt_assign(ob, order, NS, nullptr), OP_PQQR, sg.index, 1, sg.gate, units);
     };
 
     auto add_pass = [&](int kind, int flags, int op_begin, std::vector<ItemDev> items, int max_lines,
@@ -554,6 +626,23 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         int ob = (int)P.ops.size();
         for (int g = 0; g < gb_count; ++g) push_group_ops(OP_GB, gb_first + g, g, w);
         add_pass(PASS_ROW, 0, ob, chunk_rows(w), w, gb_first, gb_count);
+    }
+    if (original) {
+        // applyUnitary: the pqqp phase, then every pqqr term in map order; the
+        // alpha-pair terms (p_ alpha) precede the beta-pair ones
+        int ob = (int)P.ops.size();
+        if (!seg[kPQQP].empty()) {
+            OpDev d = blank(OP_DIAG);
+            d.index = 1;
+            P.ops.push_back(d);
+        }
+        for (const auto &sg : singles)
+            if (sg.spin == 0) push_single(sg, W);
+        if ((int)P.ops.size() > ob) add_pass(PASS_COL, 0, ob, chunk_columns(W), W, 0, 0);
+        ob = (int)P.ops.size();
+        for (const auto &sg : singles)
+            if (sg.spin == 1) push_single(sg, W);
+        if ((int)P.ops.size() > ob) add_pass(PASS_ROW, 0, ob, chunk_rows(W), W, 0, 0);
     }
     // alpha-leading double excitations, p by p: AAAA_p (COL, all columns) then ABBA_p windows
     auto max_orbit = [&](int tsize, int k0, int k1) {

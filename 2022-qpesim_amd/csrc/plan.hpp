@@ -40,7 +40,9 @@ enum OpType : int32_t {
     OP_BBBB = 4,
     OP_ABBA = 5,
     OP_FCC = 6,    // consecutive AAAA (COL pass) or BBBB (ROW pass) terms, one or more
-    OP_FAB = 7     // consecutive ABBA terms sharing (p, q, r), one or more
+    OP_FAB = 7,    // consecutive ABBA terms sharing (p, q, r), one or more
+    OP_PQQR = 8    // one pqqr term of applyUnitary's order: like FCC, on the lines whose
+                   // string has a bit of aux (aux = 0: every line)
 };
 
 // COLG: like COL, but the lines stay in global memory (windows of mixed double
@@ -59,9 +61,10 @@ enum PassFlags : int32_t {
 // to every line of the tile (FCC, GA, GB) or every column pair (FAB).
 struct OpDev {
     int32_t type;
-    int32_t index;   // FCC/FAB: first term (coefficients coef[index .. index+g)); GA/GB: offset of the pair angles
+    int32_t index;   // FCC/FAB/PQQR: first term (coefficients coef[index .. index+g)); GA/GB: offset of
+                     // the pair angles; DIAG: diagonal table set
     uint32_t tab;    // offset of the schedule
-    uint32_t aux;    // GA/GB: staged-angle slot; FAB: q | r << 8 (beta orbitals)
+    uint32_t aux;    // GA/GB: staged-angle slot; FAB: q | r << 8 (beta orbitals); PQQR: line gate mask
     int32_t lg;      // schedule rows
     int32_t ns;      // schedule slots (kThreadsPlan)
     int32_t g;       // terms in the group
@@ -129,6 +132,7 @@ struct ItemDev {
 struct Plan {
     int n = 0, e = 0;
     int order = 1;             // Trotter order (1: the reference's sequence; 2: symmetric)
+    int term_order = 0;        // 0: applyUnitary_reordered's term order; 1: applyUnitary's
     uint32_t C = 0;
     uint32_t ca_pairs = 0;     // C(n-2, e-1)
     uint32_t ccaa_pairs = 0;   // C(n-4, e-2)
@@ -136,7 +140,7 @@ struct Plan {
     size_t tile_bytes = 0;     // LDS bytes per workgroup for the amplitude tile
 
     std::vector<uint32_t> strings;      // [C] the colex-ordered strings of one spin
-    std::vector<double> theta_a, theta_b, cross;   // diag: [C], [C], [n][C]
+    std::vector<double> theta_a, theta_b, cross;   // diag, per table set: [C], [C], [n][C]
     std::vector<GroupDev> groups;
     std::vector<double> grp_a;          // GA/GB ops: [op.index + i*ns + t][2] 1-cos, sin of the pair angle part
     std::vector<double> grp_b;          // [G][C][2] 1-cos, sin of the other-string angle part
@@ -161,7 +165,12 @@ struct Plan {
 // consecutive double excitations with small joint support into one op.
 // order: 1 reproduces the reference's first-order product; 2 builds the
 // symmetric second-order step (forward sequence at t/2, then reversed at t/2).
+// term_order: 0 is applyUnitary_reordered's sequence (TEOT:374-562: pp, pqqp,
+// each pq with the pqqr terms it gates, pqrs), the one time_evolve_over_terms
+// runs; 1 is applyUnitary's (TEOT:174-372: pp, pq, pqqp, every pqqr, pqrs),
+// the one time_evolve_iqpe runs.
 // Throws std::runtime_error with a message on unsupported sizes.
-Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse = true, int order = 1);
+Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse = true, int order = 1,
+                int term_order = 0);
 
 }  // namespace qpe

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libqpesim.so")
 
 COMPLEX128, COMPLEX64 = 0, 1
 PP, PQ, PQQP, PQQR, PQRS = range(5)
-OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES, OPT_TROTTER_ORDER = 0, 1, 2, 3
+OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES, OPT_TROTTER_ORDER, OPT_TERM_ORDER = 0, 1, 2, 3, 4
 ERR_ARG, ERR_OOM, ERR_HIP, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
 
 # every entry point include/qpesim.h declares
@@ -35,6 +35,7 @@ EXPORTS = [
     "qpe_debug_pairs", "qpe_plan_info", "qpe_batch_time_evolve", "qpe_autotune", "qpe_time_evolve_blocks", "qpe_last_error",
     "qpe_hamiltonian_ingest_broombridge", "qpe_hamiltonian_ingest_broombridge_text", "qpe_hamiltonian_initial_state",
     "qpe_hamiltonian_identity_term", "qpe_hamiltonian_get_sizes", "qpe_state_ingest_broombridge", "qpe_energy_estimate",
+    "qpe_time_evolve_iqpe", "qpe_state_apply_gate", "qpe_state_apply_controlled_gate",
 ]
 BATCH_EAGER = 1
 
@@ -84,6 +85,9 @@ def load():
         "qpe_hamiltonian_get_sizes": [vp, P(ctypes.c_uint), P(ctypes.c_uint)],
         "qpe_state_ingest_broombridge": [P(vp), vp, ctypes.c_char_p, ctypes.c_uint, ci],
         "qpe_energy_estimate": [P(dbl), ctypes.c_uint, dbl, dbl, P(dbl)],
+        "qpe_time_evolve_iqpe": [vp, vp, u64],
+        "qpe_state_apply_gate": [vp, P(dbl), ctypes.c_uint],
+        "qpe_state_apply_controlled_gate": [vp, P(dbl), ctypes.c_uint, ctypes.c_uint],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -104,6 +108,32 @@ def _call(name, *args):
 
 def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _gate_words(gate):
+    g = np.asarray(gate, dtype=np.complex128).reshape(2, 2)
+    return np.ascontiguousarray(np.stack([g.real, g.imag], axis=-1).ravel())
+
+
+def hadamard():
+    """Hadamard(), cpp/gate.hpp:22-25."""
+    r = 1.0 / math.sqrt(2.0)
+    return np.array([[r, r], [r, -r]], np.complex128)
+
+
+def pauli_x():
+    """Gate1D(0, 1, 1, 0), the X of the reference's gate tests (cpp/state_vector.hpp:390)."""
+    return np.array([[0, 1], [1, 0]], np.complex128)
+
+
+def rotate(m):
+    """Rotate(m), cpp/gate.hpp:27-32: diag(1, exp(2 pi i / 2^m))."""
+    return np.array([[1, 0], [0, np.exp(2j * math.pi / 2.0 ** m)]], np.complex128)
+
+
+def inv_rotate(m):
+    """InvRotate(m), cpp/gate.hpp:34-39: diag(1, exp(-2 pi i / 2^m))."""
+    return np.array([[1, 0], [0, np.exp(-2j * math.pi / 2.0 ** m)]], np.complex128)
 
 
 class Hamiltonian:
@@ -257,6 +287,16 @@ class StateVector:
     def set_option(self, opt, value):
         _call("qpe_state_set_option", self._s, int(opt), int(value))
 
+    def apply_gate(self, gate, qubit):
+        """StateVector::apply_gate, cpp/state_vector.hpp:51-74 (gate: 2x2 complex)."""
+        g = _gate_words(gate)
+        _call("qpe_state_apply_gate", self._s, _dptr(g), int(qubit))
+
+    def apply_controlled_gate(self, gate, control, qubit):
+        """StateVector::apply_controlled_gate, cpp/state_vector.hpp:76-113."""
+        g = _gate_words(gate)
+        _call("qpe_state_apply_controlled_gate", self._s, _dptr(g), int(control), int(qubit))
+
     def qift(self):
         _call("qpe_qift", self._s)
 
@@ -289,6 +329,12 @@ def time_evolve_over_terms(sv, h):
 def time_evolve_blocks(sv, h, j_begin, j_end):
     """Blocks j_begin..j_end-1 of the chain (resumable pieces of time_evolve_over_terms)."""
     _call("qpe_time_evolve_blocks", sv.handle, h.handle, int(j_begin), int(j_end))
+
+
+def time_evolve_iqpe(sv, h, num_of_applications):
+    """time_evolve_iqpe(sv, h, k), cpp/time_evolution_over_terms.hpp:141-155:
+    applyUnitary (original term order, every pqqr term) k times in place on block 1."""
+    _call("qpe_time_evolve_iqpe", sv.handle, h.handle, int(num_of_applications))
 
 
 def batch_time_evolve(states, hams, eager=False):

@@ -134,6 +134,20 @@ int qpe_state_get(qpe_state_t s, uint64_t offset, uint64_t count, double *amplit
 /* sv[j*C^2 + i] *= factor for all i: the implicit Hadamards of cpp/main.cc:111-115 */
 int qpe_state_scale_block(qpe_state_t s, uint64_t block, double factor);
 
+/* StateVector::apply_gate(gate, qubit), cpp/state_vector.hpp:51-74: a 2x2 gate
+ * on ancilla qubit `qubit` < precision.  gate = {g00, g01, g10, g11}, each an
+ * interleaved (re, im) pair (Gate1D, cpp/gate.hpp:7-20, row-major).  Qubit q is
+ * bit (precision - q - 1) of the block index j (big-endian); for every pair of
+ * blocks j0 (bit 0), j1 (bit 1) and every amplitude a of a block:
+ * a_j0' = g00 a_j0 + g01 a_j1, a_j1' = g10 a_j0 + g11 a_j1 (FP64 arithmetic). */
+int qpe_state_apply_gate(qpe_state_t s, const double *gate, unsigned qubit);
+
+/* StateVector::apply_controlled_gate(gate, control, qubit),
+ * cpp/state_vector.hpp:76-113: the same on the block pairs whose control qubit
+ * is 1.  control != qubit, both < precision (QPE_ERR_ARG otherwise, where the
+ * reference asserts). */
+int qpe_state_apply_controlled_gate(qpe_state_t s, const double *gate, unsigned control, unsigned qubit);
+
 /* StateVector Hamiltonian::IngestBroombridge(path, precision),
  * cpp/hamiltonian.cc:8-144: qpe_hamiltonian_ingest_broombridge, then a
  * [2^precision][C][C] state of this dtype whose block 0 holds the initial
@@ -156,6 +170,15 @@ int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h);
  * Hamiltonian that ended at j_begin (QPE_ERR_STATE otherwise).  Calls covering
  * 1..2^m in order give exactly qpe_time_evolve_over_terms's result. */
 int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin, uint64_t j_end);
+
+/* time_evolve_iqpe(sv, h, num_of_applications), cpp/time_evolution_over_terms.hpp:
+ * 141-155: applyUnitary (TEOT:174-372; term order pp, pq, pqqp, every pqqr term
+ * whether or not a pq term gates it, pqrs) applied num_of_applications times in
+ * place to block 1 (the reference calls it with exp = 1), whatever
+ * QPE_OPT_TERM_ORDER says.  Block 1 is not renormalised (complex64 too); its
+ * final squared norm is block_norms()[1].  Blocks other than 1 are untouched;
+ * a later qpe_time_evolve_blocks must start at j_begin = 1. */
+int qpe_time_evolve_iqpe(qpe_state_t s, qpe_hamiltonian_t h, uint64_t num_of_applications);
 
 /* Batching form (no counterpart in the reference, whose main evolves one
  * Hamiltonian): time_evolve_over_terms for `count` independent instances in
@@ -204,10 +227,14 @@ enum qpe_option {
     QPE_OPT_ENGINE = 1,        /* 0: tile passes with fused groups of terms (default);
                                   1: tile passes with one term per op */
     QPE_OPT_MAX_TILE_BYTES = 2, /* LDS bytes a tile may use (0: device maximum) */
-    QPE_OPT_TROTTER_ORDER = 3   /* 1: the reference's first-order product (default);
+    QPE_OPT_TROTTER_ORDER = 3,  /* 1: the reference's first-order product (default);
                                    2: symmetric second order, the first-order sequence at
                                    t/2 followed by the same sequence reversed at t/2 (no
                                    counterpart in the reference) */
+    QPE_OPT_TERM_ORDER = 4      /* term order of one step of time_evolve_over_terms:
+                                   0: applyUnitary_reordered (TEOT:374-562; default, the
+                                   call TEOT:136 makes); 1: applyUnitary (TEOT:174-372, the
+                                   call left commented out at TEOT:135) */
 };
 int qpe_state_set_option(qpe_state_t s, int option, int64_t value);
 

@@ -56,6 +56,13 @@ def lib():
                                       dbl, P(dbl), i64, i64]
         L.orc_apply_unitary.restype = ctypes.c_int
         L.orc_apply_unitary.argtypes = [ctypes.c_int, ctypes.c_int, P(OrcTerm), i64, dbl, P(dbl)]
+        L.orc_apply_unitary_original.restype = ctypes.c_int
+        L.orc_apply_unitary_original.argtypes = [ctypes.c_int, ctypes.c_int, P(OrcTerm), i64, dbl, P(dbl)]
+        L.orc_time_evolve_ordered.restype = ctypes.c_int
+        L.orc_time_evolve_ordered.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(OrcTerm), i64,
+                                              dbl, P(dbl), i64, i64, ctypes.c_int]
+        L.orc_time_evolve_iqpe.restype = ctypes.c_int
+        L.orc_time_evolve_iqpe.argtypes = [ctypes.c_int, ctypes.c_int, P(OrcTerm), i64, dbl, P(dbl), i64]
         L.orc_count_updates.restype = ctypes.c_int
         L.orc_count_updates.argtypes = [ctypes.c_int, ctypes.c_int, P(OrcTerm), i64, P(u64), P(u64)]
         L.orc_qift.restype = ctypes.c_int
@@ -158,13 +165,16 @@ def expand_pairs(kind, n, e, args):
     return out
 
 
-def time_evolve(n, e, m, terms, t, state, j_begin=1, j_end=None):
-    """state: complex128 array of shape (2^m, C, C) or flat; modified in place."""
+def time_evolve(n, e, m, terms, t, state, j_begin=1, j_end=None, original=False):
+    """state: complex128 array of shape (2^m, C, C) or flat; modified in place.
+    original: each block applies applyUnitary's term order instead of
+    applyUnitary_reordered's."""
     assert state.dtype == np.complex128 and state.flags.c_contiguous
     if j_end is None:
         j_end = 1 << m
-    rc = lib().orc_time_evolve(n, e, m, terms.arr, len(terms.arr), t,
-                               state.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), j_begin, j_end)
+    rc = lib().orc_time_evolve_ordered(n, e, m, terms.arr, len(terms.arr), t,
+                                       state.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), j_begin, j_end,
+                                       1 if original else 0)
     if rc != 0:
         raise RuntimeError("orc_time_evolve failed")
     return state
@@ -176,6 +186,26 @@ def apply_unitary(n, e, terms, t, block):
                                  block.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     if rc != 0:
         raise RuntimeError("orc_apply_unitary failed")
+    return block
+
+
+def apply_unitary_original(n, e, terms, t, block):
+    """One applyUnitary (original order, every pqqr) on one block, in place."""
+    assert block.dtype == np.complex128 and block.flags.c_contiguous
+    rc = lib().orc_apply_unitary_original(n, e, terms.arr, len(terms.arr), t,
+                                          block.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc != 0:
+        raise RuntimeError("orc_apply_unitary_original failed")
+    return block
+
+
+def time_evolve_iqpe(n, e, terms, t, block, num_of_applications):
+    """time_evolve_iqpe: applyUnitary num_of_applications times on one block, in place."""
+    assert block.dtype == np.complex128 and block.flags.c_contiguous
+    rc = lib().orc_time_evolve_iqpe(n, e, terms.arr, len(terms.arr), t,
+                                    block.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(num_of_applications))
+    if rc != 0:
+        raise RuntimeError("orc_time_evolve_iqpe failed")
     return block
 
 

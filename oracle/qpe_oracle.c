@@ -690,11 +690,8 @@ static int apply_pqqr(ctx_t *x, unsigned p_, unsigned q_, unsigned r_, double c,
     return 0;
 }
 
-/* applyUnitary_reordered, TEOT:374-562 */
-static int apply_unitary(ctx_t *x, const orc_term *terms, int64_t nterms, double t) {
-    unsigned n = (unsigned)x->n, e = (unsigned)x->e;
-    int64_t b0[5] = {0, 0, 0, 0, 0}, b1[5] = {0, 0, 0, 0, 0};
-    /* terms are grouped class by class */
+/* terms are grouped class by class: [b0[c], b1[c]) holds class c */
+static void class_bounds(const orc_term *terms, int64_t nterms, int64_t *b0, int64_t *b1) {
     for (int c = 0; c < 5; ++c) {
         int64_t s = -1, z = 0;
         for (int64_t i = 0; i < nterms; ++i)
@@ -702,8 +699,12 @@ static int apply_unitary(ctx_t *x, const orc_term *terms, int64_t nterms, double
         b0[c] = s < 0 ? 0 : s;
         b1[c] = s < 0 ? 0 : z;
     }
-    /* pp: updatePP_AA | updatePP_BB, TEOT:376-395 */
-    for (int64_t i = b0[ORC_PP]; i < b1[ORC_PP]; ++i) {
+}
+
+/* pp: updatePP_AA | updatePP_BB, TEOT:376-395 (TEOT:176-195 in applyUnitary) */
+static void apply_pp_list(ctx_t *x, const orc_term *terms, int64_t i0, int64_t i1, double t) {
+    unsigned n = (unsigned)x->n;
+    for (int64_t i = i0; i < i1; ++i) {
         unsigned p = (unsigned)terms[i].idx[0];
         cplx cf = phase_coeff(terms[i].coeff, t);
         part_t *pt = NULL;
@@ -713,8 +714,12 @@ static int apply_unitary(ctx_t *x, const orc_term *terms, int64_t nterms, double
         else diag_cols(x, pt, np, cf);
         free(pt);
     }
-    /* pqqp: updatePQQP_AA | _BB | _AB, TEOT:399-424 */
-    for (int64_t i = b0[ORC_PQQP]; i < b1[ORC_PQQP]; ++i) {
+}
+
+/* pqqp: updatePQQP_AA | _BB | _AB, TEOT:399-424 (TEOT:225-250 in applyUnitary) */
+static void apply_pqqp_list(ctx_t *x, const orc_term *terms, int64_t i0, int64_t i1, double t) {
+    unsigned n = (unsigned)x->n, e = (unsigned)x->e;
+    for (int64_t i = i0; i < i1; ++i) {
         unsigned p = (unsigned)terms[i].idx[0], q = (unsigned)terms[i].idx[1];
         cplx cf = phase_coeff(terms[i].coeff, t);
         if (p < n && q < n) {
@@ -741,30 +746,31 @@ static int apply_unitary(ctx_t *x, const orc_term *terms, int64_t nterms, double
             free(pb);
         }
     }
-    /* pq, each followed by the pqqr terms whose (p_, r_) == (q, p), TEOT:428-513 */
-    for (int64_t i = b0[ORC_PQ]; i < b1[ORC_PQ]; ++i) {
-        unsigned p = (unsigned)terms[i].idx[0], q = (unsigned)terms[i].idx[1];
-        cplx ev[2], od[2];
-        rot_coeffs(terms[i].coeff, t, ev, od);
-        part_t *pt = NULL;
-        if (p < n && q < n) {
-            unsigned a[2] = {p, q};
-            int64_t np = discover(ORC_CA, x->n, x->e, a, &pt);
-            upd_pq_aa(x, pt, np, ev, od);
-        } else {
-            unsigned a[2] = {p - n, q - n};
-            int64_t np = discover(ORC_CA, x->n, x->e, a, &pt);
-            upd_pq_bb(x, pt, np, ev, od);
-        }
-        free(pt);
-        for (int64_t k = b0[ORC_PQQR]; k < b1[ORC_PQQR]; ++k) {
-            unsigned p_ = (unsigned)terms[k].idx[0], q_ = (unsigned)terms[k].idx[1];
-            unsigned r_ = (unsigned)terms[k].idx[3];
-            if (p_ == q && r_ == p) apply_pqqr(x, p_, q_, r_, terms[k].coeff, t);
-        }
+}
+
+/* one pq term: updatePQ_AA | updatePQ_BB, TEOT:428-451 (TEOT:199-221 in applyUnitary) */
+static void apply_pq(ctx_t *x, const orc_term *T, double t) {
+    unsigned n = (unsigned)x->n;
+    unsigned p = (unsigned)T->idx[0], q = (unsigned)T->idx[1];
+    cplx ev[2], od[2];
+    rot_coeffs(T->coeff, t, ev, od);
+    part_t *pt = NULL;
+    if (p < n && q < n) {
+        unsigned a[2] = {p, q};
+        int64_t np = discover(ORC_CA, x->n, x->e, a, &pt);
+        upd_pq_aa(x, pt, np, ev, od);
+    } else {
+        unsigned a[2] = {p - n, q - n};
+        int64_t np = discover(ORC_CA, x->n, x->e, a, &pt);
+        upd_pq_bb(x, pt, np, ev, od);
     }
-    /* pqrs: updatePQRS_AAAA | _BBBB | _ABBA, TEOT:519-560 */
-    for (int64_t i = b0[ORC_PQRS]; i < b1[ORC_PQRS]; ++i) {
+    free(pt);
+}
+
+/* pqrs: updatePQRS_AAAA | _BBBB | _ABBA, TEOT:519-560 (TEOT:313-357 in applyUnitary) */
+static void apply_pqrs_list(ctx_t *x, const orc_term *terms, int64_t i0, int64_t i1, double t) {
+    unsigned n = (unsigned)x->n, e = (unsigned)x->e;
+    for (int64_t i = i0; i < i1; ++i) {
         unsigned p = (unsigned)terms[i].idx[0], q = (unsigned)terms[i].idx[1];
         unsigned r = (unsigned)terms[i].idx[2], s = (unsigned)terms[i].idx[3];
         cplx ev[2], od[2];
@@ -793,6 +799,40 @@ static int apply_unitary(ctx_t *x, const orc_term *terms, int64_t nterms, double
             free(pb);
         }
     }
+}
+
+/* applyUnitary_reordered, TEOT:374-562: pp, pqqp, then each pq followed by the
+ * pqqr terms whose (p_, r_) == (q, p) (TEOT:453-465), then pqrs */
+static int apply_unitary(ctx_t *x, const orc_term *terms, int64_t nterms, double t) {
+    int64_t b0[5], b1[5];
+    class_bounds(terms, nterms, b0, b1);
+    apply_pp_list(x, terms, b0[ORC_PP], b1[ORC_PP], t);
+    apply_pqqp_list(x, terms, b0[ORC_PQQP], b1[ORC_PQQP], t);
+    for (int64_t i = b0[ORC_PQ]; i < b1[ORC_PQ]; ++i) {
+        unsigned p = (unsigned)terms[i].idx[0], q = (unsigned)terms[i].idx[1];
+        apply_pq(x, &terms[i], t);
+        for (int64_t k = b0[ORC_PQQR]; k < b1[ORC_PQQR]; ++k) {
+            unsigned p_ = (unsigned)terms[k].idx[0], q_ = (unsigned)terms[k].idx[1];
+            unsigned r_ = (unsigned)terms[k].idx[3];
+            if (p_ == q && r_ == p) apply_pqqr(x, p_, q_, r_, terms[k].coeff, t);
+        }
+    }
+    apply_pqrs_list(x, terms, b0[ORC_PQRS], b1[ORC_PQRS], t);
+    return 0;
+}
+
+/* applyUnitary, TEOT:174-372: pp, every pq, pqqp, every pqqr (TEOT:254-301,
+ * not gated by a pq term), pqrs */
+static int apply_unitary_original(ctx_t *x, const orc_term *terms, int64_t nterms, double t) {
+    int64_t b0[5], b1[5];
+    class_bounds(terms, nterms, b0, b1);
+    apply_pp_list(x, terms, b0[ORC_PP], b1[ORC_PP], t);
+    for (int64_t i = b0[ORC_PQ]; i < b1[ORC_PQ]; ++i) apply_pq(x, &terms[i], t);
+    apply_pqqp_list(x, terms, b0[ORC_PQQP], b1[ORC_PQQP], t);
+    for (int64_t k = b0[ORC_PQQR]; k < b1[ORC_PQQR]; ++k)
+        apply_pqqr(x, (unsigned)terms[k].idx[0], (unsigned)terms[k].idx[1], (unsigned)terms[k].idx[3],
+                   terms[k].coeff, t);
+    apply_pqrs_list(x, terms, b0[ORC_PQRS], b1[ORC_PQRS], t);
     return 0;
 }
 
@@ -802,9 +842,26 @@ int orc_apply_unitary(int n, int e, const orc_term *terms, int64_t n_terms, doub
     return apply_unitary(&x, terms, n_terms, t);
 }
 
-/* time_evolve_over_terms, TEOT:92-139: block j = copy(block j-1); applyUnitary_reordered */
-int orc_time_evolve(int n, int e, int m, const orc_term *terms, int64_t n_terms,
-                    double t, double *state, int64_t j_begin, int64_t j_end) {
+int orc_apply_unitary_original(int n, int e, const orc_term *terms, int64_t n_terms, double t, double *block) {
+    ctx_t x;
+    x.n = n; x.e = e; x.C = orc_nchoosek(n, e); x.blk = (cplx *)block;
+    return apply_unitary_original(&x, terms, n_terms, t);
+}
+
+/* time_evolve_iqpe, TEOT:141-155: applyUnitary(..., exp = 1) num_of_applications
+ * times, in place on block 1 (here: on the given block) */
+int orc_time_evolve_iqpe(int n, int e, const orc_term *terms, int64_t n_terms, double t, double *block,
+                         int64_t num_of_applications) {
+    if (e < 1 || n < e || num_of_applications < 0) return -1;
+    for (int64_t k = 0; k < num_of_applications; ++k)
+        if (orc_apply_unitary_original(n, e, terms, n_terms, t, block)) return -1;
+    return 0;
+}
+
+/* time_evolve_over_terms, TEOT:92-139: block j = copy(block j-1); applyUnitary_reordered
+ * (original != 0: the applyUnitary call left commented out at TEOT:135) */
+int orc_time_evolve_ordered(int n, int e, int m, const orc_term *terms, int64_t n_terms,
+                            double t, double *state, int64_t j_begin, int64_t j_end, int original) {
     if (e < 1 || n < e || m < 1 || j_begin < 1) return -1;
     uint64_t C = orc_nchoosek(n, e), D = C * C, M = 1ull << m;
     if ((uint64_t)j_end > M) return -1;
@@ -813,9 +870,14 @@ int orc_time_evolve(int n, int e, int m, const orc_term *terms, int64_t n_terms,
         memcpy(sv + (uint64_t)j * D, sv + (uint64_t)(j - 1) * D, (size_t)D * sizeof(cplx));
         ctx_t x;
         x.n = n; x.e = e; x.C = C; x.blk = sv + (uint64_t)j * D;
-        if (apply_unitary(&x, terms, n_terms, t)) return -1;
+        if ((original ? apply_unitary_original : apply_unitary)(&x, terms, n_terms, t)) return -1;
     }
     return 0;
+}
+
+int orc_time_evolve(int n, int e, int m, const orc_term *terms, int64_t n_terms,
+                    double t, double *state, int64_t j_begin, int64_t j_end) {
+    return orc_time_evolve_ordered(n, e, m, terms, n_terms, t, state, j_begin, j_end, 0);
 }
 
 static uint64_t parts_total(const part_t *p, int64_t np) {

@@ -72,9 +72,23 @@ int64_t orc_run_table(int kind, int n, int e, const int32_t *args,
 int orc_time_evolve(int n, int e, int m, const orc_term *terms, int64_t n_terms,
                     double t, double *state, int64_t j_begin, int64_t j_end);
 
+/* The same with original != 0: applyUnitary's order (TEOT:174-372) per block. */
+int orc_time_evolve_ordered(int n, int e, int m, const orc_term *terms, int64_t n_terms,
+                            double t, double *state, int64_t j_begin, int64_t j_end, int original);
+
 /* Apply exactly one applyUnitary_reordered to one block (C*C complex128). */
 int orc_apply_unitary(int n, int e, const orc_term *terms, int64_t n_terms,
                       double t, double *block);
+
+/* Apply exactly one applyUnitary (original order: pp, pq, pqqp, every pqqr,
+ * pqrs; TEOT:174-372) to one block. */
+int orc_apply_unitary_original(int n, int e, const orc_term *terms, int64_t n_terms,
+                               double t, double *block);
+
+/* time_evolve_iqpe, TEOT:141-155: num_of_applications applyUnitary calls in
+ * place on one block. */
+int orc_time_evolve_iqpe(int n, int e, const orc_term *terms, int64_t n_terms, double t, double *block,
+                         int64_t num_of_applications);
 
 /* Count det-term updates of one applyUnitary_reordered (pairs + diagonal amps). */
 int orc_count_updates(int n, int e, const orc_term *terms, int64_t n_terms,

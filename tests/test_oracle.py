@@ -244,6 +244,64 @@ def test_one_trotter_step_matches_dense_exponentials(n, e, seed):
     assert np.max(np.abs(U_or - U)) < 1e-13
 
 
+def _original_order(T):
+    """(cls, idx, coeff) in applyUnitary order (TEOT:174-372): pp, every pq,
+    pqqp, every pqqr (not gated by a pq term), pqrs."""
+    L = T.as_lists()
+    return ([(0, k, c) for k, c in L[0]] + [(1, k, c) for k, c in L[1]] + [(2, k, c) for k, c in L[2]] +
+            [(3, k, c) for k, c in L[3]] + [(4, k, c) for k, c in L[4]])
+
+
+def _dense_step(T, n, e, t, seq):
+    dim = math.comb(n, e) ** 2
+    U = np.eye(dim, dtype=np.complex128)
+    for cls, idx, c in seq:
+        U = expm(-1j * t * jw.term_generator(cls, list(idx), c, n, e)) @ U
+    return U
+
+
+@pytest.mark.parametrize("n,e,seed", [(3, 1, 21), (4, 2, 22), (4, 1, 23), (5, 2, 24)])
+def test_original_order_step_matches_dense_exponentials(n, e, seed):
+    one, two = qpesim.synthetic_integrals(n, seed)
+    T = oracle.compile_terms(n, one, two)
+    t = 0.5 / T.sum_abs()
+    U = _dense_step(T, n, e, t, _original_order(T))
+    dim = U.shape[0]
+    U_or = np.zeros((dim, dim), np.complex128)
+    for col in range(dim):
+        v = np.zeros(dim, np.complex128)
+        v[col] = 1.0
+        oracle.apply_unitary_original(n, e, T, t, v)
+        U_or[:, col] = v
+    assert np.max(np.abs(U_or - U)) < 1e-13
+    # the two orders are different products
+    U_re = _dense_step(T, n, e, t, _reference_order(T))
+    assert np.max(np.abs(U_re - U)) > 1e-6
+
+
+def test_iqpe_is_the_original_step_repeated():
+    """time_evolve_iqpe (TEOT:141-155): applyUnitary k times in place; also the
+    pqqr terms without a pq partner act (unlike the reordered step)."""
+    n, e = 4, 2
+    one, two = qpesim.synthetic_integrals(n, 31)
+    one = [x for x in one if not (x[0] == 3 and x[1] == 1)]   # drop h_31: some pqqr lose their pq
+    T = oracle.compile_terms(n, one, two)
+    t = 0.07
+    U = _dense_step(T, n, e, t, _original_order(T))
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=U.shape[0]) + 1j * rng.normal(size=U.shape[0])
+    w = v.copy()
+    oracle.time_evolve_iqpe(n, e, T, t, w, 3)
+    assert np.max(np.abs(w - U @ U @ U @ v)) < 1e-13
+    # chained form: block j = applyUnitary(block j-1)
+    m = 2
+    st = np.zeros((1 << m, U.shape[0]), np.complex128)
+    st[0] = v
+    oracle.time_evolve(n, e, m, T, t, st, original=True)
+    for j in range(1, 1 << m):
+        assert np.max(np.abs(st[j] - U @ st[j - 1])) < 1e-13
+
+
 def test_pq_gated_pqqr_quirk():
     """A pqqr term whose outer pair has no pq term is never applied (TEOT:453-465)."""
     n, e = 4, 2
