@@ -1082,10 +1082,12 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     to_device(&coef, p.coef, s);
     hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
     lds_bytes = 0;
+    pass_lds.clear();
     for (const auto &ps : p.passes) {
         if (ps.max_lines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
         const size_t tile = ps.kind == PASS_COLG ? 0 : (size_t)ps.max_lines * p.C * p.elem_bytes;
         size_t b = 4096 + tile + (size_t)ps.group_count * ps.max_lines * 16;
+        pass_lds.push_back(b);
         lds_bytes = std::max(lds_bytes, b);
     }
     hip_check(hipStreamSynchronize(s), "plan upload sync");
@@ -1138,8 +1140,10 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t j_begin, uint64
     const int norm_items = norm_items_of(p);
     for (uint64_t j = j_begin; j < j_end; ++j) {
         V *prev = base + (j - 1) * D, *cur = base + j * D;
-        for (const auto &ps : p.passes) {
-            hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.lds_bytes, s, A, ps, prev, cur, j);
+        for (size_t k = 0; k < p.passes.size(); ++k) {
+            const PassDev &ps = p.passes[k];
+            hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev, cur,
+                               j);
         }
         hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, j, norms,
                            scale, status_flag);
@@ -1160,9 +1164,9 @@ static void evolve_in_place_t(const DevicePlan &dp, void *state, uint64_t block,
     const int norm_items = norm_items_of(p);
     for (uint64_t k = 0; k < count; ++k) {
         // the step index only seeds the rounding dither (no renormalisation reads scale[])
-        for (const auto &ps : p.passes)
-            hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.lds_bytes, s, A, ps, cur, cur,
-                               k + 1);
+        for (size_t q = 0; q < p.passes.size(); ++q)
+            hipLaunchKernelGGL(pass_kernel<R>, dim3(p.passes[q].item_count), dim3(kThreads), dp.pass_lds[q], s, A,
+                               p.passes[q], cur, cur, k + 1);
         hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, block,
                            norms, scale, status_flag);
     }
@@ -1210,7 +1214,9 @@ void BatchRun::build(const std::vector<BatchMember> &members, uint64_t M_, hipSt
         L.work_off = work.size();
         L.nwork = (int)order.size();
         L.blocks = 0;
+        L.lds = 0;
         for (int i : order) {
+            L.lds = std::max(L.lds, members[i].dp->pass_lds[k]);
             BatchWork w;
             w.inst = i;
             w.block_begin = L.blocks;
@@ -1235,7 +1241,7 @@ template <typename R>
 static void batch_step(const BatchRun &b, hipStream_t s) {
     for (const auto &L : b.launches) {
         if (L.blocks == 0) continue;
-        hipLaunchKernelGGL(batch_pass_kernel<R>, dim3(L.blocks), dim3(kThreads), b.lds_bytes, s,
+        hipLaunchKernelGGL(batch_pass_kernel<R>, dim3(L.blocks), dim3(kThreads), L.lds, s,
                            reinterpret_cast<const PassArgs *>(b.args), (const BatchWork *)(b.work + L.work_off),
                            L.nwork, (const uint64_t *)b.jstep);
     }

@@ -28,7 +28,8 @@ struct DevicePlan {
     uint32_t *sched = nullptr;
     double *coef = nullptr;
     double *partials = nullptr;   // [max_items] norm partial sums
-    size_t lds_bytes = 0;         // dynamic LDS per pass launch
+    size_t lds_bytes = 0;         // largest dynamic LDS of a pass launch
+    std::vector<size_t> pass_lds; // dynamic LDS of each pass launch
 
     void upload(const Plan &p, int dtype, hipStream_t s);
     void release();
@@ -74,6 +75,7 @@ struct BatchRun {
         size_t work_off;
         int nwork;
         uint32_t blocks;
+        size_t lds;                // dynamic LDS: the largest of its members' pass k
     };
     std::vector<Launch> launches;  // one step
     hipGraphExec_t graph = nullptr;
