@@ -47,19 +47,20 @@ struct KSum {
 };
 
 // Amplitude update x + d.  complex128 rounds to nearest.  complex64 rounds
-// stochastically: d is dithered by a uniform fraction of one ulp of x (rb, a
-// signed random byte, selects it in steps of ulp/256), so the stored value is
-// unbiased.  Round to nearest would drop every update below half an ulp: the
-// large amplitudes of a state would never lose the (1 - cos) part of the small
-// rotations that feed the small ones, a norm and population bias that grows
-// step after step.
-template <typename R> __device__ inline R upd(R x, R d, int rb);
-template <> __device__ inline double upd<double>(double x, double d, int) { return x + d; }
-template <> __device__ inline float upd<float>(float x, float d, int rb) {
+// stochastically: d is dithered by a uniform fraction of one ulp of x (df, an
+// odd multiple of 1/2 in [-127.5, 127.5], selects it in steps of ulp/256), so
+// the stored value is unbiased.  Round to nearest would drop every update below
+// half an ulp: the large amplitudes of a state would never lose the (1 - cos)
+// part of the small rotations that feed the small ones, a norm and population
+// bias that grows step after step.
+template <typename R> __device__ inline R upd(R x, R d, float df);
+template <> __device__ inline double upd<double>(double x, double d, float) { return x + d; }
+template <> __device__ inline float upd<float>(float x, float d, float df) {
     const int eb = __float_as_int(x) & 0x7f800000;
     const float q = __int_as_float(max(eb - (31 << 23), 0));   // ulp(x) / 256, 0 for zero and tiny x
-    return x + fmaf((float)rb + 0.5f, q, d);
+    return x + fmaf(df, q, d);
 }
+
 __device__ inline int rbyte(uint32_t r, int k) { return (int)__builtin_amdgcn_sbfe(r, 8 * k, 8); }
 
 // 32-bit mixer for the dither bits (counter based: deterministic for a given plan)
@@ -72,16 +73,34 @@ __device__ inline uint32_t mix32(uint32_t x) {
     return x;
 }
 
+// Dither factors of one schedule entry: four signed random bytes + 1/2, drawn
+// once per (step, op, tile item, thread, entry) and shared by the rotations
+// that entry applies to the thread's lines; the k-th line of a chunk takes
+// them in the k-th cyclic order.  Every update stays unbiased (the dither is
+// independent of the data); only rounding errors within one entry correlate.
+struct Dither {
+    float a, b, c, d;
+};
+__device__ inline Dither make_dither(uint32_t seed) {
+    const uint32_t r = mix32(seed);
+    return {(float)rbyte(r, 0) + 0.5f, (float)rbyte(r, 1) + 0.5f, (float)rbyte(r, 2) + 0.5f,
+            (float)rbyte(r, 3) + 0.5f};
+}
+__device__ inline Dither turn(const Dither &z, int k) {
+    return (k & 3) == 0 ? z : (k & 3) == 1 ? Dither{z.b, z.c, z.d, z.a}
+                            : (k & 3) == 2 ? Dither{z.c, z.d, z.a, z.b} : Dither{z.d, z.a, z.b, z.c};
+}
+
 // 2x2 symmetric rotation {c, i*mu; i*mu, c} on the pair (u, v), given as
 // h = 1 - c.  The whole update -h*u + i*mu*v is formed first and added to u in
-// one rounding (upd).  r: 32 random bits (complex64 dither).
+// one rounding (upd).  z: the complex64 dither factors.
 template <typename V, typename R>
-__device__ inline void rotate(V &u, V &v, R h, R mu, uint32_t r) {
+__device__ inline void rotate(V &u, V &v, R h, R mu, const Dither &z) {
     V nu, nv;
-    nu.x = upd<R>(u.x, -h * u.x - mu * v.y, rbyte(r, 0));
-    nu.y = upd<R>(u.y, -h * u.y + mu * v.x, rbyte(r, 1));
-    nv.x = upd<R>(v.x, -h * v.x - mu * u.y, rbyte(r, 2));
-    nv.y = upd<R>(v.y, -h * v.y + mu * u.x, rbyte(r, 3));
+    nu.x = upd<R>(u.x, -h * u.x - mu * v.y, z.a);
+    nu.y = upd<R>(u.y, -h * u.y + mu * v.x, z.b);
+    nv.x = upd<R>(v.x, -h * v.x - mu * u.y, z.c);
+    nv.y = upd<R>(v.y, -h * v.y + mu * u.x, z.d);
     u = nu;
     v = nv;
 }
@@ -226,6 +245,7 @@ __device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int la
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
     const uint32_t step = (uint32_t)L * line_bytes;
     uint32_t base = (uint32_t)lane * line_bytes;
+    const Dither z = make_dither(seed);
     for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
@@ -237,7 +257,7 @@ __device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int la
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (l0 + k * L < nl) {
-                rotate(u[k], v[k], h, mu, mix32(seed + (uint32_t)(l0 + k * L) * 0x9E3779B9u));
+                rotate(u[k], v[k], h, mu, turn(z, k));
                 *reinterpret_cast<V *>(tb + base + k * step + o1) = u[k];
                 *reinterpret_cast<V *>(tb + base + k * step + o2) = v[k];
             }
@@ -252,12 +272,13 @@ __device__ inline void rotate_lines_gated(V *tile, uint32_t line_bytes, int nl, 
                                           uint32_t seed) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
-    for (int l = lane; l < nl; l += L) {
+    const Dither z = make_dither(seed);
+    for (int l = lane, k = 0; l < nl; l += L, ++k) {
         if (gate && !(line_str[l] & gate)) continue;
         V *pu = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o1);
         V *pv = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o2);
         V u = *pu, v = *pv;
-        rotate(u, v, h, mu, mix32(seed + (uint32_t)l * 0x9E3779B9u));
+        rotate(u, v, h, mu, turn(z, k));
         *pu = u;
         *pv = v;
     }
@@ -270,6 +291,7 @@ __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t os = rs * (uint32_t)sizeof(V), op = rp * (uint32_t)sizeof(V);
     const R mpos = (R)sn, mneg = (R)(-sn);
+    const Dither z = make_dither(seed);
     for (int x0 = lane; x0 < ncp; x0 += kChunk * L) {
         int2 pl[kChunk];
 #pragma unroll
@@ -285,7 +307,7 @@ __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int
         for (int k = 0; k < kChunk; ++k)
             if (x0 + k * L < ncp) {
                 const R mu = ((par ^ (uint32_t)pl[k].x) & 1u) ? mpos : mneg;
-                rotate(u[k], v[k], h, mu, mix32(seed + (uint32_t)(x0 + k * L) * 0x9E3779B9u));
+                rotate(u[k], v[k], h, mu, turn(z, k));
                 *reinterpret_cast<V *>(tb + ((uint32_t)pl[k].x & ~7u) + os) = u[k];
                 *reinterpret_cast<V *>(tb + (uint32_t)pl[k].y + op) = v[k];
             }
@@ -302,6 +324,7 @@ __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int la
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
     const uint32_t step = (uint32_t)L * line_bytes;
     uint32_t base = (uint32_t)lane * line_bytes;
+    const Dither z = make_dither(seed);
     for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
@@ -316,7 +339,7 @@ __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int la
                 const double2 b = ga[l0 + k * L];
                 const double hh = (ha + b.x) - ha * b.x + sa * b.y;
                 const double ss = (sa + b.y) - sa * b.x - ha * b.y;
-                rotate(u[k], v[k], (R)hh, (R)(par ? ss : -ss), mix32(seed + (uint32_t)(l0 + k * L) * 0x9E3779B9u));
+                rotate(u[k], v[k], (R)hh, (R)(par ? ss : -ss), turn(z, k));
                 *reinterpret_cast<V *>(tb + base + k * step + o1) = u[k];
                 *reinterpret_cast<V *>(tb + base + k * step + o2) = v[k];
             }
@@ -405,8 +428,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             V v = tile[idx], w;
             const R h = (R)(2.0 * s2 * s2), s = (R)(2.0 * s2 * c2);
             const uint32_t r = mix32(tseed + idx * 0x9E3779B9u);
-            w.x = upd<R>(v.x, s * v.y - h * v.x, rbyte(r, 0));
-            w.y = upd<R>(v.y, -s * v.x - h * v.y, rbyte(r, 1));
+            w.x = upd<R>(v.x, s * v.y - h * v.x, (float)rbyte(r, 0) + 0.5f);
+            w.y = upd<R>(v.y, -s * v.x - h * v.y, (float)rbyte(r, 1) + 0.5f);
             tile[idx] = w;
         }
     }
