@@ -314,6 +314,33 @@ __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int
     }
 }
 
+// rotate_pairs for a thread whose column pairs all fit one chunk, with the
+// pair-list entries already in registers (pl[k], valid for k < npl): the
+// pair list is read once per op instead of once per schedule entry
+template <typename R, typename V>
+__device__ inline void rotate_pairs_held(V *tile, const int2 *pl, int npl, uint32_t rs, uint32_t rp, uint32_t par,
+                                         R h, double sn, uint32_t seed) {
+    char *tb = reinterpret_cast<char *>(tile);
+    const uint32_t os = rs * (uint32_t)sizeof(V), op = rp * (uint32_t)sizeof(V);
+    const R mpos = (R)sn, mneg = (R)(-sn);
+    const Dither z = make_dither(seed);
+    V u[kChunk], v[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+        if (k < npl) {
+            u[k] = *reinterpret_cast<const V *>(tb + ((uint32_t)pl[k].x & ~7u) + os);
+            v[k] = *reinterpret_cast<const V *>(tb + (uint32_t)pl[k].y + op);
+        }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+        if (k < npl) {
+            const R mu = ((par ^ (uint32_t)pl[k].x) & 1u) ? mpos : mneg;
+            rotate(u[k], v[k], h, mu, turn(z, k));
+            *reinterpret_cast<V *>(tb + ((uint32_t)pl[k].x & ~7u) + os) = u[k];
+            *reinterpret_cast<V *>(tb + (uint32_t)pl[k].y + op) = v[k];
+        }
+}
+
 // excitation group: the angle of pair (entry, line) is the pair part (ha, sa)
 // plus the line's other-string part (gang[l]), combined by angle addition in
 // the (1 - cos, sin) form
@@ -365,6 +392,26 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             ncp = *pair_count;
         }
         if (E.lane >= E.L) return;
+        if (ncp <= kChunk * E.L) {
+            // the thread's column pairs fit one chunk: read them once for all entries
+            int2 pl[kChunk];
+            int npl = 0;
+#pragma unroll
+            for (int k = 0; k < kChunk; ++k) {
+                pl[k] = pair_list[min(E.lane + k * E.L, max(ncp - 1, 0))];
+                npl += E.lane + k * E.L < ncp ? 1 : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < kSchedMax; ++i) {
+                if (i >= d.lg) break;
+                const uint32_t e = E.e[i];
+                if (s_rank1(e) == kSchedPad) continue;
+                const double2 cf = coef[s_slot(e)];
+                rotate_pairs_held<R>(tile, pl, npl, s_rank1(e), s_rank2(e), s_par(e), (R)cf.x, cf.y,
+                                     tseed + (uint32_t)i * 0x632BE5ABu);
+            }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < kSchedMax; ++i) {
             if (i >= d.lg) break;
