@@ -262,6 +262,25 @@ def test_error_messages_name_the_line():
         qpesim.Hamiltonian().load_broombridge(text=bad)
 
 
+DUPLICATE = open(os.path.join(FIXTURES, "H2_2_sto-6g.yml")).read().replace(
+    '- [0.9950000486809176, (1a)+, (2a)+, (1b)+, (2b)+, "|vacuum>"]',
+    '- [0.9950000486809176, (1a)+, (2a)+, (1b)+, (2b)+, "|vacuum>"]\n'
+    '            - [0.5, (2a)+, (1a)+, (1b)+, (2b)+, "|vacuum>"]')
+
+
+def test_repeated_determinants_are_kept_in_file_order():
+    """Two entries for the same determinant (the second with two operators
+    swapped, so with the opposite sign): both are listed in file order; the
+    state keeps the last one (cpp/hamiltonian.cc:134-139 assigns in order)."""
+    assert DUPLICATE.count("[0.5, (2a)+") == 1
+    h = product_ingest(text=DUPLICATE)
+    addr, co = h.initial_state()
+    r = bo.ingest(DUPLICATE)
+    assert [int(a) for a in addr] == [d[0] for d in r["initial"]] == [0, 0, 7]
+    assert [float(c) for c in co] == [d[1] for d in r["initial"]]
+    assert co[1] == -0.5                       # (2a)+ (1a)+ = -(1a)+ (2a)+
+
+
 def test_ingest_accumulates_like_repeated_add_calls():
     """A second ingest adds its integrals to what the Hamiltonian holds (the
     reference's AddOneBodyTerm / AddTwoBodyTerm accumulate per key)."""

@@ -152,6 +152,21 @@ def test_qpe_peak_sits_at_the_ground_state_phase(name):
     assert res["moe"] == pytest.approx(2 * math.pi / ((1.0 / steps) * M))
 
 
+def test_repeated_determinant_keeps_the_last_coefficient(tmp_path):
+    from test_broombridge import DUPLICATE
+    path = tmp_path / "dup.yml"
+    path.write_text(DUPLICATE)
+    h = qpesim.Hamiltonian()
+    sv = h.ingest_broombridge(str(path), 2)
+    block0 = sv.get(0, sv.D)
+    sv.close()
+    want = np.zeros(block0.size, np.complex128)
+    for a, c in bo.ingest(DUPLICATE)["initial"]:
+        want[a] = c                               # later entries overwrite earlier ones
+    assert block0[0] == -0.5
+    assert np.array_equal(block0, want)
+
+
 def test_ingest_errors_leave_no_state():
     h = qpesim.Hamiltonian()
     with pytest.raises(qpesim.QpeError) as ei:
