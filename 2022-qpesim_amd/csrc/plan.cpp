@@ -509,7 +509,39 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     // the entries fill at most 3/4 of the NS x kSchedMax schedule (so the
     // packing fits).  A single term's entries are independent: a term with more
     // than NS x kSchedMax of them is split over several ops.
+    // Mixed-window runs (terms (p, q, r, s_i) sharing p, q, r): their grouping
+    // and schedules depend only on p and the s_i (the alpha side), so every run
+    // with the same alpha side reuses them, pointing at the same schedule rows.
+    struct AbOp {
+        uint32_t tab;
+        int lg, ns, g;
+        size_t first;   // the op's first term, as an offset into the run
+    };
+    std::map<std::vector<int>, std::vector<AbOp>> ab_cache;
     auto push_groups = [&](const std::vector<Rot> &list, bool ab, int units) {
+        std::vector<int> key;
+        bool consecutive = true;
+        for (size_t k = 1; k < list.size(); ++k) consecutive = consecutive && list[k].index == list[0].index + (int)k;
+        if (ab && consecutive && !list.empty()) {
+            key.push_back(units);
+            key.push_back(list[0].p);
+            for (const auto &R : list) key.push_back(R.s);
+            auto hit = ab_cache.find(key);
+            if (hit != ab_cache.end()) {
+                for (const AbOp &a : hit->second) {
+                    OpDev o = blank(OP_FAB);
+                    o.index = list[a.first].index;
+                    o.g = a.g;
+                    o.aux = (uint32_t)list[a.first].q | ((uint32_t)list[a.first].r << 8);
+                    o.tab = a.tab;
+                    o.lg = a.lg;
+                    o.ns = a.ns;
+                    P.ops.push_back(o);
+                }
+                return;
+            }
+        }
+        const size_t op0 = P.ops.size();
         auto support = [&](const Rot &R) {
             return ab ? (1u << R.p) | (1u << R.s) : (1u << R.p) | (1u << R.q) | (1u << R.r) | (1u << R.s);
         };
@@ -555,6 +587,14 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
             emit_orbits(best, order, lg512, ab ? OP_FAB : OP_FCC, grp[0].index, (int)grp.size(),
                         ab ? ((uint32_t)grp[0].q | ((uint32_t)grp[0].r << 8)) : 0u, units);
             i = j;
+        }
+        if (!key.empty()) {
+            std::vector<AbOp> rec;
+            for (size_t k = op0; k < P.ops.size(); ++k) {
+                const OpDev &o = P.ops[k];
+                rec.push_back({o.tab, o.lg, o.ns, o.g, (size_t)(o.index - list[0].index)});
+            }
+            ab_cache.emplace(std::move(key), std::move(rec));
         }
     };
     // one applyUnitary pqqr term: every entry its own orbit
@@ -731,6 +771,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         // fused group's schedule rows reversed, so every orbit sees the group's
         // terms backwards (rows of a single term or an excitation group commute)
         const size_t nf = P.passes.size();
+        std::map<uint32_t, uint32_t> reversed;   // schedule rows shared by several ops are reversed once
         for (size_t k = nf; k-- > 0;) {
             PassDev pd = P.passes[k];
             pd.flags = 0;
@@ -738,10 +779,16 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
             for (int o = P.passes[k].op_end - 1; o >= P.passes[k].op_begin; --o) {
                 OpDev op = P.ops[o];
                 if ((op.type == OP_FCC || op.type == OP_FAB) && op.g > 1) {
-                    const uint32_t ntab = (uint32_t)P.sched.size();
-                    for (int i = op.lg - 1; i >= 0; --i)
-                        for (int q = 0; q < op.ns; ++q) P.sched.push_back(P.sched[op.tab + (size_t)i * op.ns + q]);
-                    op.tab = ntab;
+                    auto hit = reversed.find(op.tab);
+                    if (hit != reversed.end()) {
+                        op.tab = hit->second;
+                    } else {
+                        const uint32_t ntab = (uint32_t)P.sched.size();
+                        for (int i = op.lg - 1; i >= 0; --i)
+                            for (int q = 0; q < op.ns; ++q) P.sched.push_back(P.sched[op.tab + (size_t)i * op.ns + q]);
+                        reversed.emplace(op.tab, ntab);
+                        op.tab = ntab;
+                    }
                 }
                 P.ops.push_back(op);
             }
