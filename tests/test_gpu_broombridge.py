@@ -173,3 +173,34 @@ def test_ingest_errors_leave_no_state():
         h.ingest_broombridge(os.path.join(FIXTURES, "PP.yml"), 3)    # one orbital: below the register's minimum
     assert ei.value.code == qpesim.ERR_ARG
     assert h.sizes() == (0, 0)
+
+
+def test_batch_of_molecules_equals_single_runs():
+    """The batching form on real molecules of different sizes (C = 6 .. 252, one
+    to three determinants): one launch sequence for all of them gives every
+    register bit for bit as its own time_evolve_over_terms."""
+    names = ["H2_2_sto-6g.yml", "h2_ccpvtz_ccsd_1_4008au.yaml", "Be-ccpVDZ-5orb.yaml",
+             "h2o_oh0.9_sto3g.nw.out.yaml", "N2-10orb.yaml", "H2O-10orb.yaml"]
+    m, t = 4, 0.1
+    say("broombridge batch of molecules")
+    for dtype in (qpesim.COMPLEX128, qpesim.COMPLEX64):
+        singles, states, hams = [], [], []
+        for name in names:
+            path = os.path.join(FIXTURES, name)
+            for batch in (False, True):
+                h = qpesim.Hamiltonian()
+                sv = h.ingest_broombridge(path, m, dtype)
+                h.index_terms()
+                h.set_timestep(t)
+                sv.scale_block(0, 1.0 / math.sqrt(1 << m))
+                if batch:
+                    states.append(sv)
+                    hams.append(h)
+                else:
+                    qpesim.time_evolve_over_terms(sv, h)
+                    singles.append(sv.get())
+                    sv.close()
+        qpesim.batch_time_evolve(states, hams)
+        for sv, ref in zip(states, singles):
+            assert np.array_equal(sv.get(), ref)
+            sv.close()
