@@ -73,18 +73,28 @@ __device__ inline uint32_t mix32(uint32_t x) {
     return x;
 }
 
-// Dither factors of one schedule entry: four signed random bytes + 1/2, drawn
-// once per (step, op, tile item, thread, entry) and shared by the rotations
-// that entry applies to the thread's lines; the k-th line of a chunk takes
-// them in the k-th cyclic order.  Every update stays unbiased (the dither is
-// independent of the data); only rounding errors within one entry correlate.
+// Dither factors of one op for one thread: four signed random bytes + 1/2 of
+// a hashed (step, op, tile item, thread) word.  Schedule entry i takes them in
+// a signed order of its own (entry_dither) and the k-th line of a chunk in the
+// k-th cyclic order of that (turn).  The factor set is symmetric, so every
+// ordering is uniform and every update stays unbiased (the dither is
+// independent of the data); only rounding errors within one op of one thread
+// correlate.
 struct Dither {
     float a, b, c, d;
 };
-__device__ inline Dither make_dither(uint32_t seed) {
-    const uint32_t r = mix32(seed);
+__device__ inline Dither make_dither(uint32_t r) {
     return {(float)rbyte(r, 0) + 0.5f, (float)rbyte(r, 1) + 0.5f, (float)rbyte(r, 2) + 0.5f,
             (float)rbyte(r, 3) + 0.5f};
+}
+__device__ inline Dither entry_dither(const Dither &z, int i) {
+    return (i & 3) == 0 ? z : (i & 3) == 1 ? Dither{-z.c, z.d, -z.a, z.b}
+                            : (i & 3) == 2 ? Dither{z.b, -z.a, z.d, -z.c} : Dither{-z.d, -z.c, z.b, z.a};
+}
+// v with its sign flipped when bit 31 of m is set (m: a parity moved to bit 31)
+__device__ inline float flip_sign(float v, uint32_t m) { return __int_as_float(__float_as_int(v) ^ (int)(m & 0x80000000u)); }
+__device__ inline double flip_sign(double v, uint32_t m) {
+    return __hiloint2double(__double2hiint(v) ^ (int)(m & 0x80000000u), __double2loint(v));
 }
 __device__ inline Dither turn(const Dither &z, int k) {
     return (k & 3) == 0 ? z : (k & 3) == 1 ? Dither{z.b, z.c, z.d, z.a}
@@ -115,7 +125,7 @@ struct PassArgs {
     const ItemDev *items;
     const uint32_t *col_list;
     const uint32_t *sched;
-    const double *coef;
+    const double *coef;   // [op][kMaxGroup][2], op-major
     double *partials;
     const double *scale;
     void *state;          // batched launches: the instance's register (column-major blocks)
@@ -165,26 +175,43 @@ __device__ inline uint32_t s_rank2(uint32_t e) { return (e >> kSchedRankBits) & 
 __device__ inline uint32_t s_slot(uint32_t e) { return (e >> 28) & 7u; }
 __device__ inline uint32_t s_par(uint32_t e) { return e >> 31; }
 
-// stage descriptors (threads 0..63) and coefficients (threads 256..319) of batch b
-__device__ inline void stage_batch(const PassArgs &A, OpDev *ring, double2 *cring, int o0, int o1, int b) {
+// Staging of batch b: descriptor words (threads 0..63) and coefficients
+// (threads 256..319, from the op-major table coef[op][kMaxGroup]) are fetched
+// into registers with independent global loads, and committed to the LDS
+// rings later, so their latency overlaps the ops of the running batch.
+struct Staged {
+    uint32_t w;
+    double2 c;
+};
+__device__ inline Staged fetch_batch(const PassArgs &A, int o0, int o1, int b) {
     const int first = o0 + b * kBatch;
     const int w = threadIdx.x;
+    Staged st{0u, make_double2(0.0, 0.0)};
     if (w < kBatch * kDescWords) {
         const int i = w / kDescWords, word = w - i * kDescWords;
-        uint32_t v = 0;
-        if (first + i < o1) v = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
-        reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[w] = v;
+        if (first + i < o1) st.w = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
     } else if (w >= 256 && w < 256 + kBatch * kMaxGroup) {
         const int i = (w - 256) / kMaxGroup, slot = (w - 256) - i * kMaxGroup;
-        double2 c = make_double2(0.0, 0.0);
         if (first + i < o1) {
-            const OpDev &od = A.ops[first + i];
-            if (od.type >= OP_AAAA && slot < od.g) {
-                const double *cf = A.coef + 2ull * (od.index + slot);
-                c = make_double2(cf[0], cf[1]);
-            }
+            const double *cf = A.coef + 2ull * ((size_t)(first + i) * kMaxGroup + slot);
+            st.c = make_double2(cf[0], cf[1]);
         }
-        cring[((b & 1) * kBatch + i) * kMaxGroup + slot] = c;
+    }
+    return st;
+}
+// The coefficient ring holds them in the state's type (complex64: rounded to
+// float once here instead of in every rotation).
+template <typename R>
+__device__ inline void commit_batch(const Staged &st, OpDev *ring, double2 *cring, int b) {
+    using V = typename Vec<R>::type;
+    const int w = threadIdx.x;
+    if (w < kBatch * kDescWords) {
+        reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[w] = st.w;
+    } else if (w >= 256 && w < 256 + kBatch * kMaxGroup) {
+        V c;
+        c.x = (R)st.c.x;
+        c.y = (R)st.c.y;
+        reinterpret_cast<V *>(cring)[(b & 1) * kBatch * kMaxGroup + (w - 256)] = c;
     }
 }
 
@@ -193,10 +220,11 @@ __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool vali
     int count = 0, stride = 0;
     E.slot = 0; E.lane = 1; E.L = 1;   // no work unless the op has a schedule
     if (valid && d.type != OP_DIAG) {
+        // lane = tid / ns and L = kThreads / ns from the descriptor's magic (exact for tid, ns <= 1024)
         const int ns = d.ns;
-        E.lane = (int)threadIdx.x / ns;
-        E.slot = (int)threadIdx.x - E.lane * ns;
-        E.L = kThreads / ns;
+        E.lane = (int)(__umul24((uint32_t)threadIdx.x, d.split & 0x1fffffu) >> 20);
+        E.slot = (int)threadIdx.x - (int)__umul24((uint32_t)E.lane, (uint32_t)ns);
+        E.L = (int)(d.split >> 21);
         if (E.lane < E.L) {
             base = A.sched + d.tab + E.slot;
             stride = ns;
@@ -217,18 +245,16 @@ __device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, co
     const int tid = threadIdx.x;
     const int q = (int)(qr & 0xff), r = (int)(qr >> 8);
     if (tid < 64) {
-        bool is01 = false;
-        int partner = -1, parb = 0;
-        if (tid < nl) {
-            uint32_t s = line_str[tid];
-            if (((s >> r) & 1u) && !((s >> q) & 1u)) {
-                uint32_t ps = s ^ (1u << q) ^ (1u << r);
-                for (int l = 0; l < nl; ++l)
-                    if (line_str[l] == ps) partner = l;
-                is01 = partner >= 0;
-                parb = popcnt(s & between_mask(q, r)) & 1;
-            }
-        }
+        // lane l holds line l's string (no string has all 32 bits set); the partner
+        // search reads the other lanes' strings with readlane, not LDS round trips
+        const uint32_t s = tid < nl ? line_str[tid] : ~0u;
+        const bool cand = tid < nl && ((s >> r) & 1u) && !((s >> q) & 1u);
+        const uint32_t ps = s ^ (1u << q) ^ (1u << r);
+        int partner = -1;
+        for (int l = 0; l < nl; ++l)
+            if ((uint32_t)__builtin_amdgcn_readlane((int)s, l) == ps) partner = l;
+        const bool is01 = cand && partner >= 0;
+        const int parb = popcnt(s & between_mask(q, r)) & 1;
         uint64_t mask = __ballot(is01);
         if (is01)
             pair_list[__popcll(mask & ((1ull << tid) - 1ull))] =
@@ -240,12 +266,11 @@ __device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, co
 // rotate (line[r1], line[r2]) on lines lane, lane + L, ... of the tile with one (h, mu)
 template <typename R, typename V>
 __device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1, uint32_t r2,
-                                    R h, R mu, uint32_t seed) {
+                                    R h, R mu, const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
     const uint32_t step = (uint32_t)L * line_bytes;
     uint32_t base = (uint32_t)lane * line_bytes;
-    const Dither z = make_dither(seed);
     for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
@@ -269,10 +294,9 @@ __device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int la
 template <typename R, typename V>
 __device__ inline void rotate_lines_gated(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1,
                                           uint32_t r2, R h, R mu, const uint32_t *line_str, uint32_t gate,
-                                          uint32_t seed) {
+                                          const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
-    const Dither z = make_dither(seed);
     for (int l = lane, k = 0; l < nl; l += L, ++k) {
         if (gate && !(line_str[l] & gate)) continue;
         V *pu = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o1);
@@ -287,11 +311,11 @@ __device__ inline void rotate_lines_gated(V *tile, uint32_t line_bytes, int nl, 
 // rotate (l01[rs], l10[rp]) on column pairs lane, lane + L, ... of the pair list
 template <typename R, typename V>
 __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int lane, int L, uint32_t rs,
-                                    uint32_t rp, uint32_t par, R h, double sn, uint32_t seed) {
+                                    uint32_t rp, uint32_t par, R h, R sn, const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t os = rs * (uint32_t)sizeof(V), op = rp * (uint32_t)sizeof(V);
-    const R mpos = (R)sn, mneg = (R)(-sn);
-    const Dither z = make_dither(seed);
+    const R mneg = -sn;   // mu = sn when the entry's and the pair's parities differ, else -sn
+    const uint32_t pm = par << 31;
     for (int x0 = lane; x0 < ncp; x0 += kChunk * L) {
         int2 pl[kChunk];
 #pragma unroll
@@ -306,7 +330,7 @@ __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int
 #pragma unroll
         for (int k = 0; k < kChunk; ++k)
             if (x0 + k * L < ncp) {
-                const R mu = ((par ^ (uint32_t)pl[k].x) & 1u) ? mpos : mneg;
+                const R mu = flip_sign(mneg, pm ^ ((uint32_t)pl[k].x << 31));
                 rotate(u[k], v[k], h, mu, turn(z, k));
                 *reinterpret_cast<V *>(tb + ((uint32_t)pl[k].x & ~7u) + os) = u[k];
                 *reinterpret_cast<V *>(tb + (uint32_t)pl[k].y + op) = v[k];
@@ -318,26 +342,26 @@ __device__ inline void rotate_pairs(V *tile, const int2 *pair_list, int ncp, int
 // pair-list entries already in registers (pl[k], valid for k < npl): the
 // pair list is read once per op instead of once per schedule entry
 template <typename R, typename V>
-__device__ inline void rotate_pairs_held(V *tile, const int2 *pl, int npl, uint32_t rs, uint32_t rp, uint32_t par,
-                                         R h, double sn, uint32_t seed) {
+__device__ inline void rotate_pairs_held(V *tile, const uint32_t *px, const uint32_t *py, const uint32_t *pb,
+                                         int npl, uint32_t rs, uint32_t rp, uint32_t par, R h, R sn,
+                                         const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t os = rs * (uint32_t)sizeof(V), op = rp * (uint32_t)sizeof(V);
-    const R mpos = (R)sn, mneg = (R)(-sn);
-    const Dither z = make_dither(seed);
+    const R mneg = -sn;
+    const uint32_t pm = par << 31;
     V u[kChunk], v[kChunk];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
         if (k < npl) {
-            u[k] = *reinterpret_cast<const V *>(tb + ((uint32_t)pl[k].x & ~7u) + os);
-            v[k] = *reinterpret_cast<const V *>(tb + (uint32_t)pl[k].y + op);
+            u[k] = *reinterpret_cast<const V *>(tb + px[k] + os);
+            v[k] = *reinterpret_cast<const V *>(tb + py[k] + op);
         }
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
         if (k < npl) {
-            const R mu = ((par ^ (uint32_t)pl[k].x) & 1u) ? mpos : mneg;
-            rotate(u[k], v[k], h, mu, turn(z, k));
-            *reinterpret_cast<V *>(tb + ((uint32_t)pl[k].x & ~7u) + os) = u[k];
-            *reinterpret_cast<V *>(tb + (uint32_t)pl[k].y + op) = v[k];
+            rotate(u[k], v[k], h, flip_sign(mneg, pm ^ pb[k]), turn(z, k));
+            *reinterpret_cast<V *>(tb + px[k] + os) = u[k];
+            *reinterpret_cast<V *>(tb + py[k] + op) = v[k];
         }
 }
 
@@ -346,12 +370,11 @@ __device__ inline void rotate_pairs_held(V *tile, const int2 *pl, int npl, uint3
 // the (1 - cos, sin) form
 template <typename R, typename V>
 __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1, uint32_t r2,
-                                    uint32_t par, double ha, double sa, const double2 *ga, uint32_t seed) {
+                                    uint32_t par, double ha, double sa, const double2 *ga, const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
     const uint32_t step = (uint32_t)L * line_bytes;
     uint32_t base = (uint32_t)lane * line_bytes;
-    const Dither z = make_dither(seed);
     for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
@@ -374,7 +397,7 @@ __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int la
 }
 
 template <typename R>
-__device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *coef, const EntSet &E,
+__device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename Vec<R>::type *coef, const EntSet &E,
                               typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
                               const uint32_t *line_str, const uint32_t *line_base, int2 *pair_list, int *pair_count,
                               int nl, uint32_t C, uint32_t &cur_qr, int &ncp, uint32_t op_seed) {
@@ -382,8 +405,9 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
     const int tid = threadIdx.x;
     const int type = d.type;
     const uint32_t line_bytes = C * (uint32_t)sizeof(V);
-    // dither bits: distinct per (step, op, tile item, thread, entry, line); op_seed carries the item
-    const uint32_t tseed = mix32(op_seed ^ (uint32_t)tid);
+    // dither: one hashed word per (step, op, tile item, thread); op_seed carries the item
+    const uint32_t tseed = op_seed ^ (uint32_t)tid;
+    const Dither z = make_dither(mix32(tseed));
     if (type == OP_FAB) {
         if (d.aux != cur_qr) {
             cur_qr = d.aux;
@@ -394,11 +418,15 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
         if (E.lane >= E.L) return;
         if (ncp <= kChunk * E.L) {
             // the thread's column pairs fit one chunk: read them once for all entries
-            int2 pl[kChunk];
+            // (byte offsets of both columns and the pair's parity in bit 31)
+            uint32_t px[kChunk], py[kChunk], pb[kChunk];
             int npl = 0;
 #pragma unroll
             for (int k = 0; k < kChunk; ++k) {
-                pl[k] = pair_list[min(E.lane + k * E.L, max(ncp - 1, 0))];
+                const int2 pl = pair_list[min(E.lane + k * E.L, max(ncp - 1, 0))];
+                px[k] = (uint32_t)pl.x & ~7u;
+                py[k] = (uint32_t)pl.y;
+                pb[k] = (uint32_t)pl.x << 31;
                 npl += E.lane + k * E.L < ncp ? 1 : 0;
             }
 #pragma unroll
@@ -406,9 +434,9 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
                 if (i >= d.lg) break;
                 const uint32_t e = E.e[i];
                 if (s_rank1(e) == kSchedPad) continue;
-                const double2 cf = coef[s_slot(e)];
-                rotate_pairs_held<R>(tile, pl, npl, s_rank1(e), s_rank2(e), s_par(e), (R)cf.x, cf.y,
-                                     tseed + (uint32_t)i * 0x632BE5ABu);
+                const V cf = coef[s_slot(e)];
+                rotate_pairs_held<R>(tile, px, py, pb, npl, s_rank1(e), s_rank2(e), s_par(e), cf.x, cf.y,
+                                     entry_dither(z, i));
             }
             return;
         }
@@ -417,9 +445,9 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             if (i >= d.lg) break;
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
-            const double2 cf = coef[s_slot(e)];
-            rotate_pairs<R>(tile, pair_list, ncp, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), (R)cf.x, cf.y,
-                            tseed + (uint32_t)i * 0x632BE5ABu);
+            const V cf = coef[s_slot(e)];
+            rotate_pairs<R>(tile, pair_list, ncp, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), cf.x, cf.y,
+                            entry_dither(z, i));
         }
     } else if (type == OP_FCC) {
         if (E.lane >= E.L) return;
@@ -428,9 +456,9 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             if (i >= d.lg) break;
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
-            const double2 cf = coef[s_slot(e)];
-            rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x,
-                            (R)(s_par(e) ? cf.y : -cf.y), tseed + (uint32_t)i * 0x632BE5ABu);
+            const V cf = coef[s_slot(e)];
+            rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), cf.x,
+                            flip_sign(cf.y, ~s_par(e) << 31), entry_dither(z, i));
         }
     } else if (type == OP_PQQR) {
         if (E.lane >= E.L) return;
@@ -439,9 +467,9 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             if (i >= d.lg) break;
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
-            const double2 cf = coef[s_slot(e)];
-            rotate_lines_gated<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), (R)cf.x,
-                                  (R)(s_par(e) ? cf.y : -cf.y), line_str, d.aux, tseed + (uint32_t)i * 0x632BE5ABu);
+            const V cf = coef[s_slot(e)];
+            rotate_lines_gated<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), cf.x,
+                                  flip_sign(cf.y, ~s_par(e) << 31), line_str, d.aux, entry_dither(z, i));
         }
     } else if (type == OP_GA || type == OP_GB) {
         if (E.lane >= E.L) return;
@@ -453,7 +481,7 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const double2 *
             if (s_rank1(e) == kSchedPad) continue;
             const double *gp = A.grp_a + 2ull * ((size_t)d.index + (size_t)i * d.ns + E.slot);
             rotate_group<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), gp[0], gp[1], ga,
-                            tseed + (uint32_t)i * 0x632BE5ABu);
+                            entry_dither(z, i));
         }
     } else if (type == OP_DIAG) {
         const uint32_t total = (uint32_t)nl * C;
@@ -609,19 +637,25 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
     int ncp = 0;
     const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
     for (int b = 0; b < nbatch; ++b) {
-        __syncthreads();
-        if (b + 1 < nbatch) stage_batch(A, ring, cring, o0, o1, b + 1);
-        __syncthreads();
+        // batch b+1 is fetched now and committed before the barrier of op kBatch/2: every
+        // wave has then left batch b-1, whose ring half it reuses, and the entries of its
+        // first op are read (at op kBatch-2) two barriers later
+        const bool more = b + 1 < nbatch;
+        Staged st{0u, make_double2(0.0, 0.0)};
+        if (more) st = fetch_batch(A, o0, o1, b + 1);
         // two ops per iteration: the even op runs on E0 while E1 loads, the odd op the other way round
         for (int i = 0; i < kBatch; i += 2) {
             const int o = o0 + b * kBatch + i;
             if (o >= o1) break;
+            if (i == kBatch / 2 && more) commit_batch<R>(st, ring, cring, b + 1);
             const OpDev *rb = ring + (b & 1) * kBatch;
-            const double2 *cb = cring + (size_t)((b & 1) * kBatch) * kMaxGroup;
+            const V *cb = reinterpret_cast<const V *>(cring) + (size_t)((b & 1) * kBatch) * kMaxGroup;
             load_entries(A, rb[i + 1], o + 1 < o1, E1);
-            op_barrier(GLOBAL);
             {
+                // descriptors are read before the barrier (their ring half was committed
+                // at least one barrier earlier)
                 const OpDev d = rb[i];
+                op_barrier(GLOBAL);
                 run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_list,
                           pair_count, nl, C, cur_qr, ncp,
                           mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
@@ -630,9 +664,9 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
             // entries of op o+2 (the next batch's first op after the last pair)
             const OpDev &dn = (i + 2 < kBatch) ? rb[i + 2] : ring[((b + 1) & 1) * kBatch];
             load_entries(A, dn, o + 2 < o1, E0);
-            op_barrier(GLOBAL);
             {
                 const OpDev d = rb[i + 1];
+                op_barrier(GLOBAL);
                 run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_list,
                           pair_count, nl, C, cur_qr, ncp,
                           mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
@@ -677,7 +711,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
         line_pos[tid] = (uint32_t)tid;
         line_str[tid] = A.strings[r];
     }
-    stage_batch(A, ring, cring, o0, o1, 0);
+    commit_batch<R>(fetch_batch(A, o0, o1, 0), ring, cring, 0);
     __syncthreads();
     for (int i = tid; i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
@@ -1145,11 +1179,27 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     to_device(&groups, p.groups, s);
     to_device(&grp_a, p.grp_a, s);
     to_device(&grp_b, p.grp_b, s);
-    to_device(&ops, p.ops, s);
+    std::vector<OpDev> ops_dev = p.ops;
+    for (OpDev &o : ops_dev) {
+        const uint32_t ns = (uint32_t)std::max(1, std::min(o.ns, kThreadsPlan));
+        o.split = ((1u << 20) + ns - 1) / ns | (uint32_t)(kThreadsPlan / ns) << 21;
+    }
+    to_device(&ops, ops_dev, s);
     to_device(&items, p.items, s);
     to_device(&col_list, p.col_list, s);
     to_device(&sched, p.sched, s);
-    to_device(&coef, p.coef, s);
+    // coefficients op-major, kMaxGroup slots per op (slot k of an op: term index + k),
+    // so that staging an op's coefficients needs no descriptor first
+    std::vector<double> opc(p.ops.size() * kMaxGroup * 2, 0.0);
+    for (size_t o = 0; o < p.ops.size(); ++o) {
+        const OpDev &od = p.ops[o];
+        if (od.type < OP_AAAA) continue;
+        for (int k = 0; k < od.g && k < kMaxGroup; ++k) {
+            opc[(o * kMaxGroup + k) * 2] = p.coef[2 * ((size_t)od.index + k)];
+            opc[(o * kMaxGroup + k) * 2 + 1] = p.coef[2 * ((size_t)od.index + k) + 1];
+        }
+    }
+    to_device(&coef, opc, s);
     hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
     lds_bytes = 0;
     pass_lds.clear();

@@ -267,7 +267,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     const size_t max_group = fuse ? (size_t)kMaxGroup : 1u;
     auto blank = [](int type) {
         OpDev o;
-        o.type = type; o.index = 0; o.tab = 0; o.aux = 0; o.lg = 0; o.ns = kThreadsPlan; o.g = 1; o.pad = 0;
+        o.type = type; o.index = 0; o.tab = 0; o.aux = 0; o.lg = 0; o.ns = kThreadsPlan; o.g = 1; o.split = 0;
         return o;
     };
     const uint32_t pad_entry = pack_sched(kSchedPad, kSchedPad, 0, 0);

@@ -68,7 +68,8 @@ struct OpDev {
     int32_t lg;      // schedule rows
     int32_t ns;      // schedule slots (kThreadsPlan)
     int32_t g;       // terms in the group
-    int32_t pad;
+    uint32_t split;  // set on upload: ceil(2^20 / ns) | (kThreadsPlan / ns) << 21, so that the
+                     // kernel splits threads into slots and lanes without a division
 };
 
 // Schedule entry: two 14-bit ranks, the term's slot in the group (3 bits) and
