@@ -47,21 +47,23 @@ struct KSum {
 };
 
 // Amplitude update x + d.  complex128 rounds to nearest.  complex64 rounds
-// stochastically: d is dithered by a uniform fraction of one ulp of x (df, an
-// odd multiple of 1/2 in [-127.5, 127.5], selects it in steps of ulp/256), so
-// the stored value is unbiased.  Round to nearest would drop every update below
-// half an ulp: the large amplitudes of a state would never lose the (1 - cos)
-// part of the small rotations that feed the small ones, a norm and population
-// bias that grows step after step.
+// stochastically: d is dithered by a uniform fraction of one ulp of x, in steps
+// of ulp/256, so the stored value is unbiased.  df is that fraction times 2^-23
+// (an odd multiple of 2^-32 in [-127.5, 127.5] * 2^-31): x with its mantissa
+// cleared is 2^e = 2^23 ulp(x), so df * 2^e is the dither (0 for zero and
+// subnormal x).  Round to nearest would drop every update below half an ulp:
+// the large amplitudes of a state would never lose the (1 - cos) part of the
+// small rotations that feed the small ones, a norm and population bias that
+// grows step after step.
 template <typename R> __device__ inline R upd(R x, R d, float df);
 template <> __device__ inline double upd<double>(double x, double d, float) { return x + d; }
 template <> __device__ inline float upd<float>(float x, float d, float df) {
-    const int eb = __float_as_int(x) & 0x7f800000;
-    const float q = __int_as_float(max(eb - (31 << 23), 0));   // ulp(x) / 256, 0 for zero and tiny x
-    return x + fmaf(df, q, d);
+    return x + fmaf(df, __int_as_float(__float_as_int(x) & 0x7f800000), d);
 }
 
 __device__ inline int rbyte(uint32_t r, int k) { return (int)__builtin_amdgcn_sbfe(r, 8 * k, 8); }
+// dither factor from byte k of r: (signed byte + 1/2) * 2^-31, the scaling upd expects
+__device__ inline float dfac(uint32_t r, int k) { return fmaf((float)rbyte(r, k), 0x1p-31f, 0x1p-32f); }
 
 // 32-bit mixer for the dither bits (counter based: deterministic for a given plan)
 __device__ inline uint32_t mix32(uint32_t x) {
@@ -74,7 +76,7 @@ __device__ inline uint32_t mix32(uint32_t x) {
 }
 
 // Dither factors of one op for one thread: four signed random bytes + 1/2 of
-// a hashed (step, op, tile item, thread) word.  Schedule entry i takes them in
+// a hashed (step, op, tile item, thread) word, scaled by 2^-31 for upd.  Schedule entry i takes them in
 // a signed order of its own (entry_dither) and the k-th line of a chunk in the
 // k-th cyclic order of that (turn).  The factor set is symmetric, so every
 // ordering is uniform and every update stays unbiased (the dither is
@@ -84,8 +86,7 @@ struct Dither {
     float a, b, c, d;
 };
 __device__ inline Dither make_dither(uint32_t r) {
-    return {(float)rbyte(r, 0) + 0.5f, (float)rbyte(r, 1) + 0.5f, (float)rbyte(r, 2) + 0.5f,
-            (float)rbyte(r, 3) + 0.5f};
+    return {dfac(r, 0), dfac(r, 1), dfac(r, 2), dfac(r, 3)};
 }
 __device__ inline Dither entry_dither(const Dither &z, int i) {
     return (i & 3) == 0 ? z : (i & 3) == 1 ? Dither{-z.c, z.d, -z.a, z.b}
@@ -503,8 +504,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
             V v = tile[idx], w;
             const R h = (R)(2.0 * s2 * s2), s = (R)(2.0 * s2 * c2);
             const uint32_t r = mix32(tseed + idx * 0x9E3779B9u);
-            w.x = upd<R>(v.x, s * v.y - h * v.x, (float)rbyte(r, 0) + 0.5f);
-            w.y = upd<R>(v.y, -s * v.x - h * v.y, (float)rbyte(r, 1) + 0.5f);
+            w.x = upd<R>(v.x, s * v.y - h * v.x, dfac(r, 0));
+            w.y = upd<R>(v.y, -s * v.x - h * v.y, dfac(r, 1));
             tile[idx] = w;
         }
     }
