@@ -216,9 +216,10 @@ __device__ inline void commit_batch(const Staged &st, OpDev *ring, double2 *crin
     }
 }
 
+static_assert(kSchedMax == 4, "load_entries loads four schedule rows");
 __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool valid, EntSet &E) {
-    const uint32_t *base = A.sched;
-    int count = 0, stride = 0;
+    uint32_t off = 0, st = 0;
+    int count = 0;
     E.slot = 0; E.lane = 1; E.L = 1;   // no work unless the op has a schedule
     if (valid && d.type != OP_DIAG) {
         // lane = tid / ns and L = kThreads / ns from the descriptor's magic (exact for tid, ns <= 1024)
@@ -227,13 +228,18 @@ __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool vali
         E.slot = (int)threadIdx.x - (int)__umul24((uint32_t)E.lane, (uint32_t)ns);
         E.L = (int)(d.split >> 21);
         if (E.lane < E.L) {
-            base = A.sched + d.tab + E.slot;
-            stride = ns;
+            off = d.tab + (uint32_t)E.slot;
+            st = (uint32_t)ns;
             count = d.lg;
         }
     }
-#pragma unroll
-    for (int i = 0; i < kSchedMax; ++i) E.e[i] = base[(i < count ? i : 0) * stride];
+    // kSchedMax loads for every thread (rows past lg repeat row 0): a static count keeps
+    // the wait counts exact; 32-bit element offsets from the table base
+    const uint32_t s1 = count > 1 ? st : 0u, s2 = count > 2 ? 2u * st : 0u, s3 = count > 3 ? 3u * st : 0u;
+    E.e[0] = A.sched[off];
+    E.e[1] = A.sched[off + s1];
+    E.e[2] = A.sched[off + s2];
+    E.e[3] = A.sched[off + s3];
 }
 
 // Column pairs of (q, r) in the tile: lines l01 (r occupied, q empty) and its
@@ -266,12 +272,13 @@ __device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, co
 
 // rotate (line[r1], line[r2]) on lines lane, lane + L, ... of the tile with one (h, mu)
 template <typename R, typename V>
-__device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1, uint32_t r2,
+__device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t base0, uint32_t r1,
+                                    uint32_t r2,
                                     R h, R mu, const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
-    const uint32_t step = (uint32_t)L * line_bytes;
-    uint32_t base = (uint32_t)lane * line_bytes;
+    const uint32_t step = __umul24((uint32_t)L, line_bytes);
+    uint32_t base = base0;   // lane * line_bytes
     for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
@@ -293,15 +300,18 @@ __device__ inline void rotate_lines(V *tile, uint32_t line_bytes, int nl, int la
 // rotate_lines on the lines whose string has a bit of gate (one pqqr term of
 // applyUnitary's order whose inner orbital lies in the other string)
 template <typename R, typename V>
-__device__ inline void rotate_lines_gated(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1,
+__device__ inline void rotate_lines_gated(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t base0,
+                                          uint32_t r1,
                                           uint32_t r2, R h, R mu, const uint32_t *line_str, uint32_t gate,
                                           const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
-    for (int l = lane, k = 0; l < nl; l += L, ++k) {
+    const uint32_t step = __umul24((uint32_t)L, line_bytes);
+    uint32_t base = base0;   // lane * line_bytes
+    for (int l = lane, k = 0; l < nl; l += L, ++k, base += step) {
         if (gate && !(line_str[l] & gate)) continue;
-        V *pu = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o1);
-        V *pv = reinterpret_cast<V *>(tb + (uint32_t)l * line_bytes + o2);
+        V *pu = reinterpret_cast<V *>(tb + base + o1);
+        V *pv = reinterpret_cast<V *>(tb + base + o2);
         V u = *pu, v = *pv;
         rotate(u, v, h, mu, turn(z, k));
         *pu = u;
@@ -370,12 +380,13 @@ __device__ inline void rotate_pairs_held(V *tile, const uint32_t *px, const uint
 // plus the line's other-string part (gang[l]), combined by angle addition in
 // the (1 - cos, sin) form
 template <typename R, typename V>
-__device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t r1, uint32_t r2,
+__device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int lane, int L, uint32_t base0, uint32_t r1,
+                                    uint32_t r2,
                                     uint32_t par, double ha, double sa, const double2 *ga, const Dither &z) {
     char *tb = reinterpret_cast<char *>(tile);
     const uint32_t o1 = r1 * (uint32_t)sizeof(V), o2 = r2 * (uint32_t)sizeof(V);
-    const uint32_t step = (uint32_t)L * line_bytes;
-    uint32_t base = (uint32_t)lane * line_bytes;
+    const uint32_t step = __umul24((uint32_t)L, line_bytes);
+    uint32_t base = base0;   // lane * line_bytes
     for (int l0 = lane; l0 < nl; l0 += kChunk * L, base += kChunk * step) {
         V u[kChunk], v[kChunk];
 #pragma unroll
@@ -401,14 +412,15 @@ template <typename R>
 __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename Vec<R>::type *coef, const EntSet &E,
                               typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
                               const uint32_t *line_str, const uint32_t *line_base, int2 *pair_list, int *pair_count,
-                              int nl, uint32_t C, uint32_t &cur_qr, int &ncp, uint32_t op_seed) {
+                              int nl, uint32_t C, uint32_t &cur_qr, int &ncp, uint32_t tseed) {
     using V = typename Vec<R>::type;
     const int tid = threadIdx.x;
     const int type = d.type;
     const uint32_t line_bytes = C * (uint32_t)sizeof(V);
-    // dither: one hashed word per (step, op, tile item, thread); op_seed carries the item
-    const uint32_t tseed = op_seed ^ (uint32_t)tid;
-    const Dither z = make_dither(mix32(tseed));
+    // dither: tseed is the thread's random word for this op (a hash of (step, op, tile
+    // item) xor a hash of the thread index)
+    const Dither z = make_dither(tseed);
+    const uint32_t lbase = __umul24((uint32_t)E.lane, line_bytes);   // the thread's first line (line-local ops)
     if (type == OP_FAB) {
         if (d.aux != cur_qr) {
             cur_qr = d.aux;
@@ -458,7 +470,7 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
             const V cf = coef[s_slot(e)];
-            rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), cf.x,
+            rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, lbase, s_rank1(e), s_rank2(e), cf.x,
                             flip_sign(cf.y, ~s_par(e) << 31), entry_dither(z, i));
         }
     } else if (type == OP_PQQR) {
@@ -469,7 +481,7 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
             const V cf = coef[s_slot(e)];
-            rotate_lines_gated<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), cf.x,
+            rotate_lines_gated<R>(tile, line_bytes, nl, E.lane, E.L, lbase, s_rank1(e), s_rank2(e), cf.x,
                                   flip_sign(cf.y, ~s_par(e) << 31), line_str, d.aux, entry_dither(z, i));
         }
     } else if (type == OP_GA || type == OP_GB) {
@@ -481,7 +493,7 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
             const uint32_t e = E.e[i];
             if (s_rank1(e) == kSchedPad) continue;
             const double *gp = A.grp_a + 2ull * ((size_t)d.index + (size_t)i * d.ns + E.slot);
-            rotate_group<R>(tile, line_bytes, nl, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), gp[0], gp[1], ga,
+            rotate_group<R>(tile, line_bytes, nl, E.lane, E.L, lbase, s_rank1(e), s_rank2(e), s_par(e), gp[0], gp[1], ga,
                             entry_dither(z, i));
         }
     } else if (type == OP_DIAG) {
@@ -636,6 +648,8 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
     const int o0 = pass.op_begin, o1 = pass.op_end;
     uint32_t cur_qr = 0xffffffffu;
     int ncp = 0;
+    // per-thread half of the dither words (the per-op half is a scalar hash)
+    const uint32_t tmix = mix32((uint32_t)threadIdx.x * 0x85EBCA77u + 0x632BE5ABu);
     const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
     for (int b = 0; b < nbatch; ++b) {
         // batch b+1 is fetched now and committed before the barrier of op kBatch/2: every
@@ -659,7 +673,7 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
                 op_barrier(GLOBAL);
                 run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_list,
                           pair_count, nl, C, cur_qr, ncp,
-                          mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u) ^ (item << 10));
+                          mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u ^ item * 0x9E3779B1u) ^ tmix);
             }
             if (o + 1 >= o1) break;
             // entries of op o+2 (the next batch's first op after the last pair)
@@ -670,7 +684,8 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
                 op_barrier(GLOBAL);
                 run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_list,
                           pair_count, nl, C, cur_qr, ncp,
-                          mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u) ^ (item << 10));
+                          mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u ^ item * 0x9E3779B1u) ^
+                              tmix);
             }
         }
     }
