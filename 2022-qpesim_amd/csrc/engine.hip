@@ -216,7 +216,6 @@ __device__ inline void commit_batch(const Staged &st, OpDev *ring, double2 *crin
     }
 }
 
-static_assert(kSchedMax == 4, "load_entries loads four schedule rows");
 __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool valid, EntSet &E) {
     uint32_t off = 0, st = 0;
     int count = 0;
@@ -235,11 +234,12 @@ __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool vali
     }
     // kSchedMax loads for every thread (rows past lg repeat row 0): a static count keeps
     // the wait counts exact; 32-bit element offsets from the table base
-    const uint32_t s1 = count > 1 ? st : 0u, s2 = count > 2 ? 2u * st : 0u, s3 = count > 3 ? 3u * st : 0u;
-    E.e[0] = A.sched[off];
-    E.e[1] = A.sched[off + s1];
-    E.e[2] = A.sched[off + s2];
-    E.e[3] = A.sched[off + s3];
+    uint32_t row = 0;
+#pragma unroll
+    for (int i = 0; i < kSchedMax; ++i) {
+        E.e[i] = A.sched[off + (i < count ? row : 0u)];
+        row += st;
+    }
 }
 
 // Column pairs of (q, r) in the tile: lines l01 (r occupied, q empty) and its
