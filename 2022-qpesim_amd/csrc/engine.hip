@@ -523,11 +523,13 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
     }
 }
 
-// Tile copies between HBM and LDS in 16-byte units, kCopyUnroll loads in
+// Tile copies between HBM and LDS in 16-byte units, copy_unroll<V>() loads in
 // flight per thread before their LDS writes.  COL tiles: whole lines (columns)
 // of C amplitudes; the line and its LDS image are 16-byte aligned when
 // C * sizeof(V) is a multiple of 16, else the copy goes element by element.
-constexpr int kCopyUnroll = 8;
+// complex64 tiles: 12 (a ten-column tile at C = 1820 is 9100 units, one round
+// for 1024 threads); complex128: 8 (register budget)
+template <typename V> constexpr int copy_unroll() { return sizeof(V) == 8 ? 12 : 8; }
 // 16-byte copy unit: a native vector type, loaded and stored as one value (a
 // struct type would be copied with memcpy, which can keep the staging array
 // out of registers)
@@ -539,21 +541,22 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // sink each load next to its write and serialise the copy.
 template <typename V, typename U>
 __device__ __forceinline__ void copy_cols_in(const V *src, V *tile, const uint32_t *line_rank, int nl, uint32_t C, double sc) {
+    constexpr int kU = copy_unroll<V>();
     const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));   // units per line
     const uint32_t total = (uint32_t)nl * upl;
     const U *s = reinterpret_cast<const U *>(src);
     U *t = reinterpret_cast<U *>(tile);
-    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
-        U r[kCopyUnroll];
-        uint32_t ii[kCopyUnroll];
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kU) {
+        U r[kU];
+        uint32_t ii[kU];
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
+        for (int k = 0; k < kU; ++k) {
             ii[k] = min(i0 + (uint32_t)k * kThreads, total - 1);
             const uint32_t l = ii[k] / upl, a = ii[k] - l * upl;
             r[k] = s[(uint64_t)line_rank[l] * upl + a];
         }
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) t[ii[k]] = r[k];
+        for (int k = 0; k < kU; ++k) t[ii[k]] = r[k];
     }
     if (sc != 1.0) {
         __syncthreads();
@@ -571,20 +574,21 @@ __device__ __forceinline__ void copy_cols_in(const V *src, V *tile, const uint32
 // into lines of C amplitudes; global runs of nl consecutive amplitudes.
 template <typename V>
 __device__ __forceinline__ void copy_rows_in(const V *src, V *tile, uint32_t r0, int nl, uint32_t C, double sc) {
+    constexpr int kU = copy_unroll<V>();
     const uint32_t total = (uint32_t)nl * C;
     const auto f = (decltype(V().x))sc;
-    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
-        V r[kCopyUnroll];
-        uint32_t tb[kCopyUnroll];
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kU) {
+        V r[kU];
+        uint32_t tb[kU];
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
+        for (int k = 0; k < kU; ++k) {
             const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
             const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
             r[k] = src[(uint64_t)b * C + r0 + l];
             tb[k] = l * C + b;
         }
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
+        for (int k = 0; k < kU; ++k) {
             V v = r[k];
             v.x *= f;
             v.y *= f;
@@ -595,40 +599,42 @@ __device__ __forceinline__ void copy_rows_in(const V *src, V *tile, uint32_t r0,
 
 template <typename V>
 __device__ __forceinline__ void copy_rows_out(V *dst, const V *tile, uint32_t r0, int nl, uint32_t C) {
+    constexpr int kU = copy_unroll<V>();
     const uint32_t total = (uint32_t)nl * C;
-    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
-        V r[kCopyUnroll];
-        uint64_t gi[kCopyUnroll];
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kU) {
+        V r[kU];
+        uint64_t gi[kU];
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
+        for (int k = 0; k < kU; ++k) {
             const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
             const uint32_t b = i / (uint32_t)nl, l = i - b * (uint32_t)nl;
             r[k] = tile[l * C + b];
             gi[k] = (uint64_t)b * C + r0 + l;
         }
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) dst[gi[k]] = r[k];
+        for (int k = 0; k < kU; ++k) dst[gi[k]] = r[k];
     }
 }
 
 template <typename V, typename U>
 __device__ __forceinline__ void copy_cols_out(V *dst, const V *tile, const uint32_t *line_rank, int nl, uint32_t C) {
+    constexpr int kU = copy_unroll<V>();
     const uint32_t upl = (uint32_t)(C * sizeof(V) / sizeof(U));
     const uint32_t total = (uint32_t)nl * upl;
     U *d = reinterpret_cast<U *>(dst);
     const U *t = reinterpret_cast<const U *>(tile);
-    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kCopyUnroll) {
-        U r[kCopyUnroll];
-        uint64_t gi[kCopyUnroll];
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kThreads * kU) {
+        U r[kU];
+        uint64_t gi[kU];
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) {
+        for (int k = 0; k < kU; ++k) {
             const uint32_t i = min(i0 + (uint32_t)k * kThreads, total - 1);
             const uint32_t l = i / upl, a = i - l * upl;
             r[k] = t[i];
             gi[k] = (uint64_t)line_rank[l] * upl + a;
         }
 #pragma unroll
-        for (int k = 0; k < kCopyUnroll; ++k) d[gi[k]] = r[k];
+        for (int k = 0; k < kU; ++k) d[gi[k]] = r[k];
     }
 }
 
