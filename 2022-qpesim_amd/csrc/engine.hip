@@ -126,7 +126,7 @@ struct PassArgs {
     const ItemDev *items;
     const uint32_t *col_list;
     const uint32_t *sched;
-    const double *coef;   // [op][kMaxGroup][2], op-major
+    const double *coef;   // [op][kMaxGroup] (1 - cos, sin), op-major, in the state's type (float pairs for complex64)
     double *partials;
     const double *scale;
     void *state;          // batched launches: the instance's register (column-major blocks)
@@ -177,44 +177,49 @@ __device__ inline uint32_t s_slot(uint32_t e) { return (e >> 28) & 7u; }
 __device__ inline uint32_t s_par(uint32_t e) { return e >> 31; }
 
 // Staging of batch b: descriptor words (threads 0..63) and coefficients
-// (threads 256..319, from the op-major table coef[op][kMaxGroup]) are fetched
-// into registers with independent global loads, and committed to the LDS
-// rings later, so their latency overlaps the ops of the running batch.
+// (threads 256..319, from the op-major table coef[op][kMaxGroup], already in
+// the state's type) are fetched into registers with independent global loads,
+// and committed to the LDS rings later, so their latency overlaps the ops of
+// the running batch (no conversion between the load and the commit, which
+// would make the fetch wait for its load).
+template <typename R>
 struct Staged {
     uint32_t w;
-    double2 c;
+    typename Vec<R>::type c;
 };
-__device__ inline Staged fetch_batch(const PassArgs &A, int o0, int o1, int b) {
+template <typename R>
+__device__ inline Staged<R> fetch_batch(const PassArgs &A, int o0, int o1, int b) {
+    using V = typename Vec<R>::type;
     const int first = o0 + b * kBatch;
-    const int w = threadIdx.x;
-    Staged st{0u, make_double2(0.0, 0.0)};
-    if (w < kBatch * kDescWords) {
-        const int i = w / kDescWords, word = w - i * kDescWords;
-        if (first + i < o1) st.w = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
-    } else if (w >= 256 && w < 256 + kBatch * kMaxGroup) {
-        const int i = (w - 256) / kMaxGroup, slot = (w - 256) - i * kMaxGroup;
-        if (first + i < o1) {
-            const double *cf = A.coef + 2ull * ((size_t)(first + i) * kMaxGroup + slot);
-            st.c = make_double2(cf[0], cf[1]);
-        }
+    // wave 0 fetches descriptor words, wave 4 coefficients: a wave-uniform branch, so
+    // the two never share a (linearised) path with a pending load
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    Staged<R> st;   // each wave sets only the field it commits
+    if (wave == 0) {
+        const int i = lane / kDescWords, word = lane - i * kDescWords;
+        uint32_t v = 0u;
+        if (first + i < o1) v = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
+        st.w = v;
+    } else if (wave == 4) {
+        const int i = lane / kMaxGroup, slot = lane - i * kMaxGroup;
+        V c;
+        c.x = 0;
+        c.y = 0;
+        if (first + i < o1) c = reinterpret_cast<const V *>(A.coef)[(size_t)(first + i) * kMaxGroup + slot];
+        st.c = c;
     }
     return st;
 }
-// The coefficient ring holds them in the state's type (complex64: rounded to
-// float once here instead of in every rotation).
 template <typename R>
-__device__ inline void commit_batch(const Staged &st, OpDev *ring, double2 *cring, int b) {
+__device__ inline void commit_batch(const Staged<R> &st, OpDev *ring, double2 *cring, int b) {
     using V = typename Vec<R>::type;
-    const int w = threadIdx.x;
-    if (w < kBatch * kDescWords) {
-        reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[w] = st.w;
-    } else if (w >= 256 && w < 256 + kBatch * kMaxGroup) {
-        V c;
-        c.x = (R)st.c.x;
-        c.y = (R)st.c.y;
-        reinterpret_cast<V *>(cring)[(b & 1) * kBatch * kMaxGroup + (w - 256)] = c;
-    }
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    if (wave == 0) reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[lane] = st.w;
+    else if (wave == 4) reinterpret_cast<V *>(cring)[(b & 1) * kBatch * kMaxGroup + lane] = st.c;
 }
+static_assert(kBatch * kDescWords == 64 && kBatch * kMaxGroup == 64, "one wave stages each ring");
 
 __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool valid, EntSet &E) {
     uint32_t off = 0, st = 0;
@@ -662,8 +667,8 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
         // wave has then left batch b-1, whose ring half it reuses, and the entries of its
         // first op are read (at op kBatch-2) two barriers later
         const bool more = b + 1 < nbatch;
-        Staged st{0u, make_double2(0.0, 0.0)};
-        if (more) st = fetch_batch(A, o0, o1, b + 1);
+        Staged<R> st;
+        if (more) st = fetch_batch<R>(A, o0, o1, b + 1);
         // two ops per iteration: the even op runs on E0 while E1 loads, the odd op the other way round
         for (int i = 0; i < kBatch; i += 2) {
             const int o = o0 + b * kBatch + i;
@@ -733,7 +738,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
         line_pos[tid] = (uint32_t)tid;
         line_str[tid] = A.strings[r];
     }
-    commit_batch<R>(fetch_batch(A, o0, o1, 0), ring, cring, 0);
+    commit_batch<R>(fetch_batch<R>(A, o0, o1, 0), ring, cring, 0);
     __syncthreads();
     for (int i = tid; i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
@@ -1211,7 +1216,8 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     to_device(&col_list, p.col_list, s);
     to_device(&sched, p.sched, s);
     // coefficients op-major, kMaxGroup slots per op (slot k of an op: term index + k),
-    // so that staging an op's coefficients needs no descriptor first
+    // so that staging an op's coefficients needs no descriptor first; stored in the
+    // state's type (complex64: rounded to float here, once)
     std::vector<double> opc(p.ops.size() * kMaxGroup * 2, 0.0);
     for (size_t o = 0; o < p.ops.size(); ++o) {
         const OpDev &od = p.ops[o];
@@ -1221,7 +1227,15 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
             opc[(o * kMaxGroup + k) * 2 + 1] = p.coef[2 * ((size_t)od.index + k) + 1];
         }
     }
-    to_device(&coef, opc, s);
+    if (dt == 1) {
+        std::vector<float> opf(opc.size());
+        for (size_t k = 0; k < opc.size(); ++k) opf[k] = (float)opc[k];
+        float *f = nullptr;
+        to_device(&f, opf, s);
+        coef = reinterpret_cast<double *>(f);
+    } else {
+        to_device(&coef, opc, s);
+    }
     hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
     lds_bytes = 0;
     pass_lds.clear();

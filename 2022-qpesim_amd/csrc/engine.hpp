@@ -26,7 +26,8 @@ struct DevicePlan {
     ItemDev *items = nullptr;
     uint32_t *col_list = nullptr;
     uint32_t *sched = nullptr;
-    double *coef = nullptr;      // [op][kMaxGroup][2]: (1 - cos, sin) of each op's terms
+    double *coef = nullptr;      // [op][kMaxGroup][2]: (1 - cos, sin) of each op's terms, in the
+                                 // state's type (float pairs for complex64)
     double *partials = nullptr;   // [max_items] norm partial sums
     size_t lds_bytes = 0;         // largest dynamic LDS of a pass launch
     std::vector<size_t> pass_lds; // dynamic LDS of each pass launch
