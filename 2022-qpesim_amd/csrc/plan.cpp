@@ -287,12 +287,6 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         for (auto &sl : slot) lg = std::max(lg, sl.size());
         return (int)lg;
     };
-    // Work split of an op: ns schedule slots; the L = NS / ns threads of a slot
-    // share its lines (or column pairs).  Cost model: rows x units per thread.
-    auto split_cost = [&](int lg, int ns, int units) {
-        const int L = NS / ns;
-        return (double)lg * (double)((units + L - 1) / L) + 0.01 * lg;
-    };
     // excitation group g: its pairs k in slot k % ns, row k / ns, split into
     // ops of at most kSchedMax rows; the pair angles are laid out the same way
     auto push_group_ops = [&](int type, int g, int stage_slot, int units) {
@@ -479,18 +473,41 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     // may exceed it at any slot count: its rows are then split over ops).
     auto emit_orbits = [&](const Orbits &best, const std::vector<uint32_t> &order, int lg512, int type, int index,
                            int g, uint32_t aux, int units) {
-        int ns_best = NS;
-        double bc = split_cost(std::min(lg512, kSchedMax), NS, units) * ((lg512 + kSchedMax - 1) / kSchedMax);
         size_t total = 0, longest = 0;
         orbit_stats(best, total, longest);
-        for (int L = 2; L <= std::min(NS, units); ++L) {   // more lanes than units idle
-            const int cand = NS / L;
-            if ((size_t)cand * kSchedMax < total || longest > (size_t)kSchedMax) break;
+        // Work split: ns schedule slots, the L = NS / ns threads of a slot share its lines
+        // (or column pairs); threads past ns * L, and lanes past the units, idle.
+        // VALU model: a wave runs lg rows of its busiest thread's units (lane l of L
+        // takes units l, l + L, ...), the SIMDs (wave w on SIMD w % 4) run their waves'
+        // instructions one after another, and the busiest SIMD sets the op's time; the
+        // longest single chain breaks ties
+        auto vcost = [&](int ns, int lg) {
+            const int L = NS / ns;
+            int simd[4] = {0, 0, 0, 0}, crit = 0;
+            for (int w = 0; w < NS / 64; ++w) {
+                const int lane = (w * 64) / ns;   // a wave's first thread has its busiest lane
+                const int per = lane < L && lane < units ? (units - lane + L - 1) / L : 0;
+                simd[w % 4] += per * lg;
+                crit = std::max(crit, per * lg);
+            }
+            return (double)std::max(std::max(simd[0], simd[1]), std::max(simd[2], simd[3])) + 0.01 * crit;
+        };
+        int ns_best = NS;
+        double bc = vcost(NS, std::min(lg512, kSchedMax)) * ((lg512 + kSchedMax - 1) / kSchedMax);
+        // candidates: NS / L for each useful lane count, one orbit per slot, and the
+        // fewest slots for each row count (no padded slots)
+        std::vector<int> cands;
+        for (int L = 2; L <= std::min(NS, units); ++L) cands.push_back(NS / L);
+        cands.push_back((int)best.size());
+        for (size_t t = std::max<size_t>(longest, 1); t <= (size_t)kSchedMax; ++t) cands.push_back((int)((total + t - 1) / t));
+        for (int cand : cands) {
+            if (cand < 1 || cand > NS) continue;
+            if ((size_t)cand * kSchedMax < total || longest > (size_t)kSchedMax) continue;
             const int lower = (int)std::max<size_t>(longest, (total + cand - 1) / cand);
-            if (split_cost(lower, cand, units) >= bc - 1e-9) continue;
+            if (vcost(cand, lower) >= bc - 1e-9) continue;
             const int lgc = lpt_assign(best, order, cand, nullptr);
             if (lgc > kSchedMax) continue;
-            const double c = split_cost(lgc, cand, units);
+            const double c = vcost(cand, lgc);
             if (c < bc - 1e-9) { bc = c; ns_best = cand; }
         }
         Slots slot = pack_slots(best, order, ns_best);
