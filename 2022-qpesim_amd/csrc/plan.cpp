@@ -477,12 +477,18 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         orbit_stats(best, total, longest);
         // Work split: ns schedule slots, the L = NS / ns threads of a slot share its lines
         // (or column pairs); threads past ns * L, and lanes past the units, idle.
-        // VALU model: a wave runs lg rows of its busiest thread's units (lane l of L
-        // takes units l, l + L, ...), the SIMDs (wave w on SIMD w % 4) run their waves'
-        // instructions one after another, and the busiest SIMD sets the op's time; the
-        // longest single chain breaks ties
+        // With several units per slot, a VALU model: a wave runs lg rows of its busiest
+        // thread's units (lane l of L takes units l, l + L, ...), the SIMDs (wave w on
+        // SIMD w % 4) run their waves' instructions one after another, and the busiest
+        // SIMD sets the op's time; the longest single chain breaks ties.  Candidates: NS / L
+        // for each useful lane count, one orbit per slot, and the fewest slots for each row
+        // count (no padded slots).  With one unit per slot (e.g. three-column windows) the
+        // per-thread chain matters more than the padding (measured: fewer, fuller waves
+        // were 4% slower there), so only NS / L slots compete, on rows x units per thread.
+        const bool model = units >= 2;
         auto vcost = [&](int ns, int lg) {
             const int L = NS / ns;
+            if (!model) return (double)lg * ((units + L - 1) / L) + 0.01 * lg;
             int simd[4] = {0, 0, 0, 0}, crit = 0;
             for (int w = 0; w < NS / 64; ++w) {
                 const int lane = (w * 64) / ns;   // a wave's first thread has its busiest lane
@@ -494,12 +500,13 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         };
         int ns_best = NS;
         double bc = vcost(NS, std::min(lg512, kSchedMax)) * ((lg512 + kSchedMax - 1) / kSchedMax);
-        // candidates: NS / L for each useful lane count, one orbit per slot, and the
-        // fewest slots for each row count (no padded slots)
         std::vector<int> cands;
         for (int L = 2; L <= std::min(NS, units); ++L) cands.push_back(NS / L);
-        cands.push_back((int)best.size());
-        for (size_t t = std::max<size_t>(longest, 1); t <= (size_t)kSchedMax; ++t) cands.push_back((int)((total + t - 1) / t));
+        if (model) {
+            cands.push_back((int)best.size());
+            for (size_t t = std::max<size_t>(longest, 1); t <= (size_t)kSchedMax; ++t)
+                cands.push_back((int)((total + t - 1) / t));
+        }
         for (int cand : cands) {
             if (cand < 1 || cand > NS) continue;
             if ((size_t)cand * kSchedMax < total || longest > (size_t)kSchedMax) continue;

@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per pass_kernel<float> launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
 # each scaled by the same counter's reading for kernels of known size (FETCH x1.997, WRITE x1.0);
 # see profiles/pass_kernel_n32N8m2_pmc_traffic.md.  Keyed by (spin-orbitals, electrons, dtype, order).
-MEASURED_TRAFFIC = {(32, 8, "c64", 1): 43.33e6}
+MEASURED_TRAFFIC = {(32, 8, "c64", 1): 43.22e6}
 
 
 def parse():
