@@ -55,10 +55,12 @@ enum PassFlags : int32_t {
     PF_EMIT_NORM = 2   // write per-workgroup FP64 partial sums of |a|^2
 };
 
-// Every op but DIAG carries a schedule: kThreadsPlan slots (one per thread of
-// the pass kernel) of lg <= kSchedMax entries, entry i of slot t at
-// sched[tab + i*kThreadsPlan + t].  A thread applies its entries in order, each
-// to every line of the tile (FCC, GA, GB) or every column pair (FAB).
+// Every op but DIAG carries a schedule: ns <= kThreadsPlan slots of
+// lg <= kSchedMax entries, entry i of slot t at sched[tab + i*ns + t].  Thread
+// x of the pass kernel serves slot x % ns as lane x / ns of L = kThreadsPlan / ns
+// (threads past ns * L idle); it applies its slot's entries in order, each to
+// its share (lane, lane + L, ...) of the tile's lines (FCC, GA, GB, PQQR) or
+// column pairs (FAB).
 struct OpDev {
     int32_t type;
     int32_t index;   // FCC/FAB/PQQR: first term (coefficients coef[index .. index+g)); GA/GB: offset of
