@@ -160,7 +160,7 @@ __device__ inline void op_barrier(bool global_tile) {
 // previous op runs, into two alternating register sets.
 constexpr int kBatch = 8;
 constexpr int kDescWords = sizeof(OpDev) / 4;   // 8
-constexpr int kChunk = 4;                       // lines (column pairs) whose LDS loads are issued together
+constexpr int kChunk = 2;                       // lines (column pairs) whose LDS loads are issued together
 static_assert(kThreads == kThreadsPlan, "schedules are laid out for the pass kernel's thread count");
 
 // Schedule entries of one op for this thread.  The op's ns slots each get
