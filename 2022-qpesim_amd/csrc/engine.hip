@@ -705,7 +705,7 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
 }
 
 // One tile item of one pass: load the item's lines, run the pass's ops, write back.
-template <typename R>
+template <typename R, int MODE>
 __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass, uint32_t item,
                                           typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
     using V = typename Vec<R>::type;
@@ -722,7 +722,8 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     double2 *cring = reinterpret_cast<double2 *>(smem + 1808);          // [2][kBatch][kMaxGroup]
     static_assert(16 * kWaves <= 8 * kMaxLines, "norm reduction does not fit the pair list");
     static_assert(1808 + 2 * kBatch * kMaxGroup * 16 <= 4096, "LDS header overlaps the tile");
-    const bool gmode = pass.kind == PASS_COLG;   // lines stay in global memory
+    // lines stay in global memory (MODE 1: never, 2: always, 0: as the pass says)
+    const bool gmode = MODE == 2 ? true : MODE == 1 ? false : pass.kind == PASS_COLG;
     V *tile = reinterpret_cast<V *>(smem + 4096);
     uint32_t *line_pos = reinterpret_cast<uint32_t *>(smem + 1040);     // [kMaxLines] LDS line index (0..63)
 
@@ -814,10 +815,12 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     }
 }
 
-template <typename R>
+// MODE 1: LDS-tiled passes only, MODE 2: global-memory windows only (separate
+// kernels keep the common one small)
+template <typename R, int MODE>
 __global__ void __launch_bounds__(kThreads)
 pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
-    pass_body<R>(A, pass, blockIdx.x, prev, cur, j);
+    pass_body<R, MODE>(A, pass, blockIdx.x, prev, cur, j);
 }
 
 // Pointers loaded from a table carry no address space, so the compiler would
@@ -882,7 +885,7 @@ batch_pass_kernel(const PassArgs *__restrict__ args, const BatchWork *__restrict
     const PassArgs A = from_global_args(reinterpret_cast<const PassArgsG *>(args)[bw.inst]);
     const uint64_t j = *jp;
     V *base = reinterpret_cast<V *>(A.state);
-    pass_body<R>(A, bw.pass, blockIdx.x - bw.block_begin, base + (j - 1) * A.D, base + j * A.D, j);
+    pass_body<R, 0>(A, bw.pass, blockIdx.x - bw.block_begin, base + (j - 1) * A.D, base + j * A.D, j);
 }
 
 // sum the per-workgroup partials of block j; set norms[j] and scale[j]
@@ -1293,15 +1296,21 @@ static void evolve_t(const DevicePlan &dp, void *state, uint64_t j_begin, uint64
     const PassArgs A = make_args(dp, state, scale, renorm);
     const uint64_t D = A.D;
     V *base = reinterpret_cast<V *>(state);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     const int norm_items = norm_items_of(p);
     for (uint64_t j = j_begin; j < j_end; ++j) {
         V *prev = base + (j - 1) * D, *cur = base + j * D;
         for (size_t k = 0; k < p.passes.size(); ++k) {
             const PassDev &ps = p.passes[k];
-            hipLaunchKernelGGL(pass_kernel<R>, dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev, cur,
-                               j);
+            if (ps.kind == PASS_COLG)
+                hipLaunchKernelGGL((pass_kernel<R, 2>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps,
+                                   prev, cur, j);
+            else
+                hipLaunchKernelGGL((pass_kernel<R, 1>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps,
+                                   prev, cur, j);
         }
         hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, j, norms,
                            scale, status_flag);
@@ -1317,14 +1326,20 @@ static void evolve_in_place_t(const DevicePlan &dp, void *state, uint64_t block,
     const Plan &p = dp.plan;
     const PassArgs A = make_args(dp, state, scale, 0);
     V *cur = reinterpret_cast<V *>(state) + block * A.D;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     const int norm_items = norm_items_of(p);
     for (uint64_t k = 0; k < count; ++k) {
         // the step index only seeds the rounding dither (no renormalisation reads scale[])
         for (size_t q = 0; q < p.passes.size(); ++q)
-            hipLaunchKernelGGL(pass_kernel<R>, dim3(p.passes[q].item_count), dim3(kThreads), dp.pass_lds[q], s, A,
-                               p.passes[q], cur, cur, k + 1);
+            if (p.passes[q].kind == PASS_COLG)
+                hipLaunchKernelGGL((pass_kernel<R, 2>), dim3(p.passes[q].item_count), dim3(kThreads), dp.pass_lds[q],
+                                   s, A, p.passes[q], cur, cur, k + 1);
+            else
+                hipLaunchKernelGGL((pass_kernel<R, 1>), dim3(p.passes[q].item_count), dim3(kThreads), dp.pass_lds[q],
+                                   s, A, p.passes[q], cur, cur, k + 1);
         hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, block,
                            norms, scale, status_flag);
     }
