@@ -262,7 +262,7 @@ def main():
     achieved = pass_bytes / passes / avg_launch_s / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((2 * n, 2 * e, "c64", 1)),
-                       "kernel": "pass_kernel<float> (term-rotation tile pass)",
+                       "kernel": "pass_kernel<float, 1> (term-rotation tile pass)",
                        "bytes_per_launch": pass_bytes / passes, "avg_launch_us": avg_launch_s * 1e6}
 
     if a.ref_pk:
