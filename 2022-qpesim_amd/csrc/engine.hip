@@ -1,6 +1,8 @@
 // gfx950 kernels of the QPE hot path and their launchers.
 //
-//   pass_kernel   one tile pass of a Trotter step (COL or ROW tiles in LDS)
+//   pass_kernel   one tile pass of a Trotter step: <R, 1> COL or ROW tiles in
+//                 LDS, <R, 2> mixed windows rotated in global memory (COLG);
+//                 batch_pass_kernel runs either kind for a batch of instances
 //   norm_finalize FP64 compensated block norm + renormalisation factor
 //   qift_kernel   batched radix-2 DFT along the ancilla axis in LDS, with the
 //                 1/sqrt(M) normalisation and the P(k) partial sums fused
