@@ -332,14 +332,14 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         return idx;
     };
     auto lpt_assign = [](const Orbits &orbits, const std::vector<uint32_t> &order, int ns, std::vector<int> *where) {
-        bool unit = true;
-        for (const auto &o : orbits) unit = unit && o.size() == 1;
-        if (unit) {   // one entry per orbit: round robin is the LPT packing
+        bool equal = true;
+        for (const auto &o : orbits) equal = equal && o.size() == orbits[0].size();
+        if (equal) {   // orbits of one length: round robin is the LPT packing (ties go to the lowest slot)
             if (where) {
                 where->assign(orbits.size(), 0);
                 for (size_t r = 0; r < order.size(); ++r) (*where)[order[r]] = (int)(r % ns);
             }
-            return (int)((orbits.size() + ns - 1) / ns);
+            return orbits.empty() ? 0 : (int)((orbits.size() + ns - 1) / ns * orbits[0].size());
         }
         std::vector<std::pair<size_t, int>> heap;   // (load, slot), min-heap
         heap.reserve(ns);
