@@ -120,3 +120,14 @@ def test_batch_instances_follow_the_spec():
     for i, x in enumerate(inst):
         assert 6 <= x["n"] <= 12 and x["e"] in (2, 3, 4) and 2 * x["e"] <= x["n"]
         assert x["seed"] == 1000 + i and x["m"] == 8
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with a message (one GPU per box; no multi-GPU path)."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
