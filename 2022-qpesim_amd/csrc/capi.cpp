@@ -2,6 +2,7 @@
 // mapping and the device-side StateVector.  No exceptions cross the ABI.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -57,16 +58,23 @@ uint64_t choose(int n, int k) {
     return qpe::binom(tab, n, k);
 }
 
+// Process-wide generation numbers: every Hamiltonian mutation, state and device
+// plan gets a value no other object ever had, so caches keyed on them cannot
+// mistake a new object for a freed one that lived at the same address.
+std::atomic<uint64_t> g_generation{1};
+uint64_t next_generation() { return g_generation.fetch_add(1, std::memory_order_relaxed); }
+
 }  // namespace
 
 struct qpe_hamiltonian {
     qpe::Hamiltonian h;
-    uint64_t version = 0;
+    uint64_t version = next_generation();   // changes (to a fresh generation) on every mutation
     double identity = 0.0;                    // GetIdentityTerm
     std::vector<qpe::Determinant> initial;    // initial state of the last Broombridge ingest
 };
 
 struct qpe_state {
+    const uint64_t id = next_generation();
     unsigned m = 0, n = 0, e = 0;
     int dtype = QPE_COMPLEX128;
     uint32_t C = 0;
@@ -90,8 +98,9 @@ struct qpe_state {
     double t_evolve = 0, t_qift = 0, t_pk = 0;
     int passes = 0, launches = 0;
     std::unique_ptr<qpe::DevicePlan> plan;
+    uint64_t plan_id = 0;                  // generation of the device plan (0: none)
     const qpe_hamiltonian *plan_h = nullptr;
-    uint64_t plan_version = ~0ull;
+    uint64_t plan_version = ~0ull;         // the Hamiltonian generation the plan was built from
     int max_lds = 0;
 
     ~qpe_state() {
@@ -132,7 +141,7 @@ void adopt_broombridge(qpe_hamiltonian *h, qpe::BroombridgeData &&d) {
     qpe::apply_broombridge(d, h->h);
     h->identity = d.identity;
     h->initial = std::move(d.initial);
-    ++h->version;
+    h->version = next_generation();
 }
 
 void set_ones(double *dev, uint64_t count, hipStream_t st) {
@@ -151,6 +160,28 @@ void materialize_scale(qpe_state_t s) {
         if (sc[j] != 1.0) qpe::scale_block(s->dtype, s->amp, j, s->D, sc[j], s->stream);
     set_ones(s->scale, s->M, s->stream);
     s->scale_pending = false;
+}
+
+// The last batch's device tables and captured graph, reused while the members,
+// their plans and the flags are unchanged.  The key holds generation numbers
+// (state ids, plan ids), never addresses; a member's destruction or new plan
+// drops the cached run at once, so no graph outlives the buffers it points at.
+struct BatchCache {
+    std::vector<uint64_t> key;             // (state id, plan id, renorm) per member
+    std::unique_ptr<qpe::BatchRun> run;
+};
+BatchCache g_batch;
+std::mutex g_batch_mu;
+
+void drop_batch_with(uint64_t state_id) {
+    std::lock_guard<std::mutex> lock(g_batch_mu);
+    if (!g_batch.run) return;
+    for (size_t i = 0; i < g_batch.key.size(); i += 3)
+        if (g_batch.key[i] == state_id) {
+            g_batch.run.reset();
+            g_batch.key.clear();
+            return;
+        }
 }
 
 }  // namespace
@@ -176,7 +207,7 @@ int qpe_hamiltonian_add_one_body_term(qpe_hamiltonian_t h, int p, int q, double 
         require(p >= 1 && q >= 1 && p >= q, QPE_ERR_ARG, "one-body indices must be 1-based with p >= q");
         require(std::isfinite(coeff), QPE_ERR_ARG, "non-finite coefficient");
         h->h.AddOneBodyTerm(p, q, coeff);
-        ++h->version;
+        h->version = next_generation();
     });
 }
 
@@ -186,7 +217,7 @@ int qpe_hamiltonian_add_two_body_term(qpe_hamiltonian_t h, int p, int q, int r, 
         require(p >= 1 && q >= 1 && r >= 1 && s >= 1, QPE_ERR_ARG, "two-body indices must be 1-based");
         require(std::isfinite(coeff), QPE_ERR_ARG, "non-finite coefficient");
         h->h.AddTwoBodyTerm(p, q, r, s, coeff);
-        ++h->version;
+        h->version = next_generation();
     });
 }
 
@@ -195,7 +226,7 @@ int qpe_hamiltonian_set_num_orbitals(qpe_hamiltonian_t h, unsigned n) {
         check_ham(h);
         require(n >= 1 && n <= 32, QPE_ERR_ARG, "num_orbitals must be in [1, 32]");
         h->h.SetNumOrbitals(n);
-        ++h->version;
+        h->version = next_generation();
     });
 }
 
@@ -203,7 +234,7 @@ int qpe_hamiltonian_set_num_occupied(qpe_hamiltonian_t h, unsigned e) {
     return guarded([&] {
         check_ham(h);
         h->h.SetNumOccupied(e);
-        ++h->version;
+        h->version = next_generation();
     });
 }
 
@@ -212,7 +243,7 @@ int qpe_hamiltonian_index_terms(qpe_hamiltonian_t h) {
         check_ham(h);
         require(h->h.num_orbitals() >= 1, QPE_ERR_STATE, "set num_orbitals before IndexTerms");
         h->h.IndexTerms();
-        ++h->version;
+        h->version = next_generation();
     });
 }
 
@@ -221,7 +252,7 @@ int qpe_hamiltonian_set_timestep(qpe_hamiltonian_t h, double t) {
         check_ham(h);
         require(std::isfinite(t), QPE_ERR_ARG, "non-finite timestep");
         h->h.SetTimestep(t);
-        ++h->version;
+        h->version = next_generation();
     });
 }
 
@@ -347,7 +378,9 @@ int qpe_state_create(qpe_state_t *out, unsigned precision, unsigned num_orbitals
 
 int qpe_state_destroy(qpe_state_t s) {
     return guarded([&] {
-        if (s) (void)hipStreamSynchronize(s->stream);
+        if (!s) return;
+        (void)hipStreamSynchronize(s->stream);
+        drop_batch_with(s->id);   // a cached batch graph must not outlive this state's buffers
         delete s;
     });
 }
@@ -548,7 +581,9 @@ static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h, int term_order) {
         }
         std::unique_ptr<qpe::DevicePlan> dp(new qpe::DevicePlan());
         dp->upload(p, s->dtype == QPE_COMPLEX128 ? 0 : 1, s->stream);
+        drop_batch_with(s->id);   // the cached batch graph points at the plan being replaced
         s->plan = std::move(dp);
+        s->plan_id = next_generation();
         s->plan_h = h;
         s->plan_version = h->version;
         s->plan_term_order = term_order;
@@ -633,17 +668,6 @@ int qpe_time_evolve_iqpe(qpe_state_t s, qpe_hamiltonian_t h, uint64_t num_of_app
 
 // ---- batched evolution --------------------------------------------------------
 
-namespace {
-// the last batch's device tables and graph, reused while the members, their
-// plans and the flags are unchanged
-struct BatchCache {
-    std::vector<const void *> key;
-    std::unique_ptr<qpe::BatchRun> run;
-};
-BatchCache g_batch;
-std::mutex g_batch_mu;
-}  // namespace
-
 int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, int flags,
                           double *elapsed_ms) {
     return guarded([&] {
@@ -658,11 +682,11 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
         for (int i = 0; i < count; ++i) prepare_evolve(states[i], hams[i], states[i]->term_order);
         for (int i = 0; i < count; ++i) qpe::hip_check(hipStreamSynchronize(states[i]->stream), "batch prepare sync");
         std::lock_guard<std::mutex> lock(g_batch_mu);
-        std::vector<const void *> key;
+        std::vector<uint64_t> key;
         for (int i = 0; i < count; ++i) {
-            key.push_back(states[i]);
-            key.push_back(states[i]->plan.get());
-            key.push_back(reinterpret_cast<const void *>((uintptr_t)states[i]->renorm));
+            key.push_back(states[i]->id);
+            key.push_back(states[i]->plan_id);
+            key.push_back((uint64_t)states[i]->renorm);
         }
         if (!g_batch.run || g_batch.key != key) {
             std::vector<qpe::BatchMember> mem;
