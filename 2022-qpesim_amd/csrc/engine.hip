@@ -27,8 +27,10 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-constexpr int kThreads = kThreadsPlan;   // threads per pass workgroup (16 waves)
+constexpr int kThreads = kThreadsPlan;   // threads per pass workgroup
 constexpr int kWaves = kThreads / 64;
+// the wave that stages coefficients (wave 0 stages descriptors)
+constexpr int kCoefWave = kWaves >= 8 ? 4 : kWaves / 2;
 constexpr int kMaxLines = 64;
 constexpr int kBinomWords = kBinomRows * kBinomCols;
 
@@ -195,7 +197,7 @@ template <typename R>
 __device__ inline Staged<R> fetch_batch(const PassArgs &A, int o0, int o1, int b) {
     using V = typename Vec<R>::type;
     const int first = o0 + b * kBatch;
-    // wave 0 fetches descriptor words, wave 4 coefficients: a wave-uniform branch, so
+    // wave 0 fetches descriptor words, wave kCoefWave coefficients: a wave-uniform branch, so
     // the two never share a (linearised) path with a pending load
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
@@ -205,7 +207,7 @@ __device__ inline Staged<R> fetch_batch(const PassArgs &A, int o0, int o1, int b
         uint32_t v = 0u;
         if (first + i < o1) v = reinterpret_cast<const uint32_t *>(A.ops + first + i)[word];
         st.w = v;
-    } else if (wave == 4) {
+    } else if (wave == kCoefWave) {
         const int i = lane / kMaxGroup, slot = lane - i * kMaxGroup;
         V c;
         c.x = 0;
@@ -221,7 +223,7 @@ __device__ inline void commit_batch(const Staged<R> &st, OpDev *ring, double2 *c
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
     if (wave == 0) reinterpret_cast<uint32_t *>(ring + (b & 1) * kBatch)[lane] = st.w;
-    else if (wave == 4) reinterpret_cast<V *>(cring)[(b & 1) * kBatch * kMaxGroup + lane] = st.c;
+    else if (wave == kCoefWave) reinterpret_cast<V *>(cring)[(b & 1) * kBatch * kMaxGroup + lane] = st.c;
 }
 static_assert(kBatch * kDescWords == 64 && kBatch * kMaxGroup == 64, "one wave stages each ring");
 

@@ -80,7 +80,12 @@ constexpr uint32_t kSchedRankBits = 14;
 constexpr uint32_t kSchedPad = (1u << kSchedRankBits) - 1u;
 constexpr int kMaxGroup = 8;
 constexpr int kSchedMax = 4;        // schedule rows per op (entries per thread)
-constexpr int kThreadsPlan = 1024;  // threads per workgroup of the pass kernel
+// threads per workgroup of the pass kernel (a build-time choice: 256, 512 or 1024)
+#ifndef QPE_PASS_THREADS
+#define QPE_PASS_THREADS 1024
+#endif
+constexpr int kThreadsPlan = QPE_PASS_THREADS;
+static_assert(kThreadsPlan == 256 || kThreadsPlan == 512 || kThreadsPlan == 1024, "pass kernel workgroup size");
 constexpr int kFuseSupportCC = 7;   // max spin-orbitals touched by a fused AAAA/BBBB group
 constexpr int kFuseSupportAB = 5;   // max alpha orbitals touched by a fused ABBA group
 inline uint32_t pack_sched(uint32_t r1, uint32_t r2, uint32_t slot, uint32_t par) {

@@ -100,13 +100,10 @@ def test_broombridge_qpe_fp64_matches_oracle(name, m, steps):
 @pytest.mark.slow
 def test_broombridge_qpe_at_full_block_size():
     """H2O in 14 orbitals with 5+5 electrons: C = 2002, the block size of the
-    n=28 / N=10 workload, from a real molecule (three Trotter steps).
-
-    At t = 0.5 / sum|c| (the synthetic workloads' step) the FP64 bar holds.  At
-    the reference program's t = 1/steps = 0.1 one step turns sum|c| t = 55 rad
-    (against 0.5), and the two FP64 evaluations (fused angle sums in the
-    (1 - cos, sin) form here, cos / sin per term in the oracle) drift apart by a
-    few 1e-12 in P(k): the bar there is 1e-11."""
+    n=28 / N=10 workload, from a real molecule (three Trotter steps), at the
+    synthetic workloads' step t = 0.5 / sum|c| and at the reference program's
+    t = 1/steps = 0.1, where one step turns sum|c| t = 55 rad.  The FP64 bar
+    holds at both (measured: 5e-15 relative L2 per block at t = 0.1)."""
     path = os.path.join(FIXTURES, "H2O-14orb.yaml")
     m = 2
     h = qpesim.Hamiltonian()
@@ -121,8 +118,8 @@ def test_broombridge_qpe_at_full_block_size():
     say("broombridge fp64 H2O-14orb m=2 t=0.1")
     amps, P, _ = product_run(path, m, 10)
     _, ra, rP = oracle_run(path, m, 10)
-    assert rel_l2(amps, ra) < 1e-11
-    assert np.max(np.abs(P - rP)) < 1e-11
+    assert rel_l2(amps, ra) < 1e-12
+    assert np.max(np.abs(P - rP)) < 1e-12
 
 
 @pytest.mark.parametrize("name", ["Be-ccpVDZ-5orb.yaml", "h2o_oh0.9_sto3g.nw.out.yaml", "H2O-10orb.yaml"])
