@@ -810,6 +810,26 @@ int qpe_qift(qpe_state_t s) {
     });
 }
 
+int qpe_qift_noalloc(qpe_state_t s, double *probs_device) {
+    return guarded([&] {
+        check_state(s);
+        require(probs_device != nullptr, QPE_ERR_ARG, "null probs");
+        require(s->M <= qpe::kNoAllocScratchDoubles, QPE_ERR_ARG,
+                "2^precision exceeds the allocation-free QIFT's limit of 2^20 outcomes");
+        hipPointerAttribute_t at;
+        std::memset(&at, 0, sizeof(at));
+        const hipError_t pe = hipPointerGetAttributes(&at, probs_device);
+        if (pe != hipSuccess) (void)hipGetLastError();   // the failed query must not surface in a later check
+        require(pe == hipSuccess && (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged), QPE_ERR_ARG,
+                "probs must point to device memory (2^precision doubles)");
+        qpe::qift_noalloc(s->dtype, s->amp, (int)s->m, s->D, s->scale, s->twiddles, (size_t)s->max_lds, probs_device,
+                          s->stream);
+        qpe::hip_check(hipStreamSynchronize(s->stream), "qift sync");
+        s->scale_pending = false;   // the QIFT applied the factors and reset them to 1
+        s->pk_valid = false;        // no partial sums of this QIFT are kept with the state
+    });
+}
+
 int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs) {
     return guarded([&] {
         check_state(s);

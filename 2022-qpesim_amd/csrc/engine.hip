@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1510,7 +1511,7 @@ void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, d
 }
 
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial, int *grid_out,
-          double *twiddles, size_t max_lds, hipStream_t s) {
+          double *twiddles, size_t max_lds, hipStream_t s, int max_grid) {
     const uint32_t M = 1u << m;
     const size_t tw_bytes = (size_t)std::max<uint32_t>(1, M / 2) * sizeof(double2);
     int w = 1;
@@ -1521,7 +1522,8 @@ void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double
     if (tw_bytes + (size_t)M * w * sizeof(double2) > max_lds || (size_t)M * w > (size_t)kQThreads * kQAcc)
         throw std::runtime_error("2^m too large for the QIFT tile");
     const uint64_t ntiles = (D + w - 1) / w;
-    int grid = (int)std::min<uint64_t>(ntiles, 1024);
+    int grid = (int)std::min<uint64_t>(ntiles, (uint64_t)std::min(1024, max_grid));
+    if (grid < 1) throw std::runtime_error("no workgroup fits the P(k) partial sums");
     size_t lds = tw_bytes + (size_t)M * w * sizeof(double2);
     *grid_out = grid;
     if (dtype == 0) {
@@ -1536,6 +1538,53 @@ void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double
                            D, m, w, scale, reinterpret_cast<const double2 *>(twiddles), pk_partial);
     }
     hip_check(hipGetLastError(), "qift launch");
+}
+
+// ---- allocation-free QIFT + P(k) ---------------------------------------------
+// The per-workgroup P(k) partial sums of qift_noalloc live in this file-scope
+// array (one copy per device, held until the process ends), so that entry point
+// allocates nothing.  Host code never passes the array's name to a kernel: it
+// uses the device address hipGetSymbolAddress returns, looked up once per device.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+__global__ void fill_kernel(double *p, uint64_t n, double v) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) p[i] = v;
+}
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// device address of `scratch` on the current device
+double *scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxScratchDevices) throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void qift_noalloc(int dtype, void *state, int m, uint64_t D, double *scale, double *twiddles, size_t max_lds,
+                  double *probs_dev, hipStream_t s) {
+    const uint64_t M = 1ull << m;
+    if (M > kNoAllocScratchDoubles)
+        throw std::runtime_error("2^m above the allocation-free QIFT's scratch (" +
+                                 std::to_string(kNoAllocScratchDoubles) + " P(k) partial sums)");
+    double *part = scratch_address();
+    int grid = 0;
+    // every workgroup writes all 2^m of its partial sums, so the scratch needs no clearing
+    qift(dtype, state, m, D, scale, part, &grid, twiddles, max_lds, s, (int)(kNoAllocScratchDoubles / M));
+    reduce_pk(part, grid, m, probs_dev, s);
+    // the QIFT applied the renormalisation factors: reset them to 1 (a fill kernel, no memset)
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)std::min<uint64_t>((M + 255) / 256, 1024)), dim3(256), 0, s, scale,
+                       M, 1.0);
+    hip_check(hipGetLastError(), "scale reset launch");
 }
 
 void reduce_pk(const double *pk_partial, int grid, int m, double *probs_dev, hipStream_t s) {

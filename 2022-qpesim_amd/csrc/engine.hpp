@@ -109,8 +109,18 @@ void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, d
                 double *partials, int nparts, hipStream_t s);
 
 // QIFT in place with the renormalisation factors applied, and P(k) partials.
+// max_grid caps the workgroups (one row of 2^m partial sums each in pk_partial).
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial,
-          int *grid_out, double *twiddles, size_t max_lds, hipStream_t s);
+          int *grid_out, double *twiddles, size_t max_lds, hipStream_t s, int max_grid = 1024);
+
+// The same QIFT and P(k) with no allocation and no host transfer: the partial sums
+// in a static device array of kNoAllocScratchDoubles doubles (one per device), P(k)
+// written to probs_dev (device memory, 2^m doubles), scale[] reset to 1 by a fill
+// kernel.  Calls on one device share the array: overlapping calls race.
+constexpr uint64_t kNoAllocScratchDoubles = 1ull << 20;   // 8 MiB
+constexpr int kMaxScratchDevices = 64;
+void qift_noalloc(int dtype, void *state, int m, uint64_t D, double *scale, double *twiddles, size_t max_lds,
+                  double *probs_dev, hipStream_t s);
 void reduce_pk(const double *pk_partial, int grid, int m, double *probs_dev, hipStream_t s);
 
 // reference-order host data <-> device layout, with dtype conversion

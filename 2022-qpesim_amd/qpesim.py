@@ -35,7 +35,7 @@ EXPORTS = [
     "qpe_debug_pairs", "qpe_plan_info", "qpe_batch_time_evolve", "qpe_autotune", "qpe_time_evolve_blocks", "qpe_last_error",
     "qpe_hamiltonian_ingest_broombridge", "qpe_hamiltonian_ingest_broombridge_text", "qpe_hamiltonian_initial_state",
     "qpe_hamiltonian_identity_term", "qpe_hamiltonian_get_sizes", "qpe_state_ingest_broombridge", "qpe_energy_estimate",
-    "qpe_time_evolve_iqpe", "qpe_state_apply_gate", "qpe_state_apply_controlled_gate",
+    "qpe_time_evolve_iqpe", "qpe_state_apply_gate", "qpe_state_apply_controlled_gate", "qpe_qift_noalloc",
 ]
 BATCH_EAGER = 1
 
@@ -88,6 +88,7 @@ def load():
         "qpe_time_evolve_iqpe": [vp, vp, u64],
         "qpe_state_apply_gate": [vp, P(dbl), ctypes.c_uint],
         "qpe_state_apply_controlled_gate": [vp, P(dbl), ctypes.c_uint, ctypes.c_uint],
+        "qpe_qift_noalloc": [vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -299,6 +300,11 @@ class StateVector:
 
     def qift(self):
         _call("qpe_qift", self._s)
+
+    def qift_noalloc(self, probs_device_ptr):
+        """qpe_qift_noalloc: QIFT + P(k) into caller-owned device memory
+        (probs_device_ptr: address of 2^m doubles on the device), no allocation."""
+        _call("qpe_qift_noalloc", self._s, ctypes.c_void_p(int(probs_device_ptr)))
 
     def identify_phase_amplitudes(self):
         out = np.zeros(self.M, np.float64)

@@ -210,6 +210,19 @@ int qpe_qift(qpe_state_t s);
  * probs[k] = sum_d |y_k(d)|^2, k < 2^m (FP64, compensated sums). */
 int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs);
 
+/* No counterpart in the reference.  For callers that cannot allocate memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): qpe_qift followed by qpe_identify_phase_amplitudes, with the same
+ * kernels and results, but no allocation and no host transfer.  probs_device is
+ * caller-owned DEVICE memory of 2^precision doubles (QPE_ERR_ARG otherwise); it
+ * receives P(k).  The per-workgroup P(k) partial sums live in a static device
+ * array of 2^20 doubles (8 MiB per device, held until the process ends); states
+ * of 2^precision > 2^20 outcomes are rejected (QPE_ERR_ARG).  Calls that overlap
+ * on one device share that array and race: serialise them.  Afterwards the state
+ * holds the QIFT output and its renormalisation factors are 1; a later
+ * qpe_identify_phase_amplitudes computes P(k) from the register itself. */
+int qpe_qift_noalloc(qpe_state_t s, double *probs_device);
+
 /* Phase post-processing of cpp/main.cc:149-167 (host only): phase =
  * (sum_k k*probs[k]) / 2^precision, summed in k order;
  * out[0] = -2*pi*phase + identity, the reference's energy estimate (it does not
