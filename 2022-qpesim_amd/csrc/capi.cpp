@@ -77,8 +77,10 @@ struct qpe_state {
     const uint64_t id = next_generation();
     unsigned m = 0, n = 0, e = 0;
     int dtype = QPE_COMPLEX128;
-    uint32_t C = 0;
-    uint64_t D = 0, M = 0;
+    int layout = QPE_LAYOUT_ALPHA_BETA;
+    uint32_t C = 0;            // strings per spin, C(n, e)
+    uint64_t D = 0, M = 0;     // amplitudes per block (C^2, or C(2n, 2e) single-string), blocks
+    void *ws = nullptr;        // single-string layout: two alpha x beta blocks the passes work on
     size_t elem = 16;
     void *amp = nullptr;
     double *norms = nullptr, *scale = nullptr, *pk_partial = nullptr, *twiddles = nullptr, *probs = nullptr;
@@ -105,7 +107,7 @@ struct qpe_state {
 
     ~qpe_state() {
         plan.reset();
-        void *ptrs[] = {amp, norms, scale, pk_partial, twiddles, probs, norm_partials, status};
+        void *ptrs[] = {amp, norms, scale, pk_partial, twiddles, probs, norm_partials, status, ws};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -327,6 +329,11 @@ int qpe_hamiltonian_get_sizes(qpe_hamiltonian_t h, unsigned *num_orbitals, unsig
 }
 
 int qpe_state_create(qpe_state_t *out, unsigned precision, unsigned num_orbitals, unsigned num_occupied, int dtype) {
+    return qpe_state_create_layout(out, precision, num_orbitals, num_occupied, dtype, QPE_LAYOUT_ALPHA_BETA);
+}
+
+int qpe_state_create_layout(qpe_state_t *out, unsigned precision, unsigned num_orbitals, unsigned num_occupied,
+                            int dtype, int layout) {
     return guarded([&] {
         require(out != nullptr, QPE_ERR_ARG, "null output pointer");
         *out = nullptr;
@@ -334,11 +341,18 @@ int qpe_state_create(qpe_state_t *out, unsigned precision, unsigned num_orbitals
         require(num_orbitals >= 2 && num_orbitals <= 32, QPE_ERR_ARG, "num_orbitals must be in [2, 32]");
         require(num_occupied >= 1 && num_occupied < num_orbitals, QPE_ERR_ARG, "need 1 <= num_occupied < num_orbitals");
         require(dtype == QPE_COMPLEX128 || dtype == QPE_COMPLEX64, QPE_ERR_ARG, "unknown dtype");
+        require(layout == QPE_LAYOUT_ALPHA_BETA || layout == QPE_LAYOUT_SINGLE_STRING, QPE_ERR_ARG, "unknown layout");
         uint64_t C = choose((int)num_orbitals, (int)num_occupied);
         require(C >= 1 && C <= 65535, QPE_ERR_ARG, "C(num_orbitals, num_occupied) exceeds 65535");
+        uint64_t D = C * C;
+        if (layout == QPE_LAYOUT_SINGLE_STRING) {
+            // 2n spin-orbitals: ranks of strings of up to 34 bits (the binomial table's rows)
+            require(num_orbitals <= 17, QPE_ERR_ARG, "the single-string layout holds up to 17 spatial orbitals");
+            D = choose(2 * (int)num_orbitals, 2 * (int)num_occupied);
+        }
         std::unique_ptr<qpe_state> s(new qpe_state());
-        s->m = precision; s->n = num_orbitals; s->e = num_occupied; s->dtype = dtype;
-        s->C = (uint32_t)C; s->D = C * C; s->M = 1ull << precision;
+        s->m = precision; s->n = num_orbitals; s->e = num_occupied; s->dtype = dtype; s->layout = layout;
+        s->C = (uint32_t)C; s->D = D; s->M = 1ull << precision;
         s->elem = dtype == QPE_COMPLEX128 ? 16 : 8;
         s->renorm = dtype == QPE_COMPLEX64 ? 1 : 0;
         const uint64_t bytes = s->M * s->D * s->elem;
@@ -408,7 +422,10 @@ int qpe_state_set(qpe_state_t s, uint64_t offset, uint64_t count, const double *
                 uint64_t c = std::min(chunk, count - done);
                 qpe::hip_check(hipMemcpyAsync(staging, amps + 2 * done, c * 2 * sizeof(double), hipMemcpyHostToDevice,
                                               s->stream), "upload");
-                qpe::scatter_in(s->dtype, s->amp, staging, offset + done, c, s->C, s->stream);
+                if (s->layout == QPE_LAYOUT_SINGLE_STRING)
+                    qpe::single_string_copy(s->dtype, s->amp, staging, offset + done, c, true, s->stream);
+                else
+                    qpe::scatter_in(s->dtype, s->amp, staging, offset + done, c, s->C, s->stream);
             }
             qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
         } catch (...) {
@@ -433,7 +450,10 @@ int qpe_state_get(qpe_state_t s, uint64_t offset, uint64_t count, double *amps) 
         try {
             for (uint64_t done = 0; done < count; done += chunk) {
                 uint64_t c = std::min(chunk, count - done);
-                qpe::gather_out(s->dtype, s->amp, staging, offset + done, c, s->C, s->stream);
+                if (s->layout == QPE_LAYOUT_SINGLE_STRING)
+                    qpe::single_string_copy(s->dtype, s->amp, staging, offset + done, c, false, s->stream);
+                else
+                    qpe::gather_out(s->dtype, s->amp, staging, offset + done, c, s->C, s->stream);
                 qpe::hip_check(hipMemcpyAsync(amps + 2 * done, staging, c * 2 * sizeof(double), hipMemcpyDeviceToHost,
                                               s->stream), "download");
                 qpe::hip_check(hipStreamSynchronize(s->stream), "sync");
@@ -626,11 +646,43 @@ int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin,
                         s->plan_term_order == s->term_order,
                     QPE_ERR_STATE, "blocks before j_begin have not been evolved with this Hamiltonian");
         }
+        const bool k1 = s->layout == QPE_LAYOUT_SINGLE_STRING;
+        if (k1) {
+            if (!s->ws)
+                qpe::hip_check(hipMalloc(&s->ws, 2 * (size_t)s->C * s->C * s->elem), "hipMalloc(alpha x beta blocks)");
+            if (j_begin == 1) {
+                // the passes evolve the (e alpha, e beta) sector only
+                qpe::hip_check(hipMemsetAsync(s->status, 0, sizeof(int), s->stream), "zero status");
+                qpe::single_string_outside(*s->plan, s->amp, s->D, s->status, s->stream);
+                int outside = 0;
+                qpe::hip_check(hipMemcpy(&outside, s->status, sizeof(int), hipMemcpyDeviceToHost), "status copy");
+                require(outside == 0, QPE_ERR_ARG,
+                        "block 0 has amplitudes outside the sector of num_occupied alpha and beta electrons");
+            }
+        }
         hipEvent_t e0, e1;
         qpe::hip_check(hipEventCreate(&e0), "event");
         qpe::hip_check(hipEventCreate(&e1), "event");
         qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
-        qpe::evolve(*s->plan, s->amp, j_begin, j_end, s->norms, s->scale, s->renorm, s->status, s->stream, &s->launches);
+        if (k1) {
+            // block j-1 in alpha x beta order, one step, block j back in single-string order;
+            // the two alpha x beta blocks swap roles, so each block is gathered once
+            char *wsb = static_cast<char *>(s->ws);
+            const size_t bb = (size_t)s->C * s->C * s->elem;
+            char *amp = static_cast<char *>(s->amp);
+            int cur = 0;
+            qpe::single_string_convert(*s->plan, wsb, amp + (j_begin - 1) * s->D * s->elem, true, s->stream);
+            for (uint64_t j = j_begin; j < j_end; ++j) {
+                qpe::evolve_step(*s->plan, wsb + cur * bb, wsb + (1 - cur) * bb, j, s->norms, s->scale, s->renorm,
+                                 s->status, s->stream);
+                qpe::single_string_convert(*s->plan, wsb + (1 - cur) * bb, amp + j * s->D * s->elem, false, s->stream);
+                cur = 1 - cur;
+            }
+            s->launches = (int)s->plan->plan.passes.size() + 2;
+        } else {
+            qpe::evolve(*s->plan, s->amp, j_begin, j_end, s->norms, s->scale, s->renorm, s->status, s->stream,
+                        &s->launches);
+        }
         qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
         qpe::hip_check(hipEventSynchronize(e1), "evolve sync");
         float ms = 0.f;
@@ -645,6 +697,7 @@ int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin,
 int qpe_time_evolve_iqpe(qpe_state_t s, qpe_hamiltonian_t h, uint64_t num_of_applications) {
     return guarded([&] {
         check_state(s);
+        require(s->layout == QPE_LAYOUT_ALPHA_BETA, QPE_ERR_ARG, "time_evolve_iqpe needs the alpha x beta layout");
         prepare_evolve(s, h, 1);
         hipEvent_t e0, e1;
         qpe::hip_check(hipEventCreate(&e0), "event");
@@ -678,6 +731,7 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
             for (int k = 0; k < i; ++k) require(states[k] != states[i], QPE_ERR_ARG, "a state appears twice in the batch");
             require(states[i]->dtype == states[0]->dtype, QPE_ERR_ARG, "batch members must share the amplitude type");
             require(states[i]->M == states[0]->M, QPE_ERR_ARG, "batch members must share the ancilla count");
+            require(states[i]->layout == QPE_LAYOUT_ALPHA_BETA, QPE_ERR_ARG, "batch members need the alpha x beta layout");
         }
         for (int i = 0; i < count; ++i) prepare_evolve(states[i], hams[i], states[i]->term_order);
         for (int i = 0; i < count; ++i) qpe::hip_check(hipStreamSynchronize(states[i]->stream), "batch prepare sync");

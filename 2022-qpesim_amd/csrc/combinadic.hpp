@@ -80,6 +80,60 @@ QPE_HD uint32_t between_mask(int a, int b) {
 
 QPE_HD int popcnt(uint32_t x) { return __builtin_popcount(x); }
 
+// Single-string (interleaved) layout.  Spin-orbital 2p is alpha orbital p, 2p+1
+// is beta orbital p; a determinant is one string of 2n bits, stored at its colex
+// rank among the strings with the same number of electrons.  The reference's
+// blocked order puts alpha orbitals before beta ones, so the same creation
+// operators in ascending order differ by one swap per (alpha a, beta b) pair
+// with b < a: amplitude(interleaved) = sign * amplitude(blocked).
+
+// C(i, k) for any k: the table holds k < kBinomCols, and C(i, k) = C(i, i - k)
+// brings every k of a row i < kBinomRows = 2 * kBinomCols - 1 into it
+QPE_HD uint32_t binom_sym(const uint32_t *tab, int i, int k) {
+    if (k > i - k) k = i - k;
+    return binom(tab, i, k);
+}
+
+// colex rank of the interleaved string of (alpha string sa, beta string sb)
+QPE_HD uint32_t interleaved_rank(uint32_t sa, uint32_t sb, int n, const uint32_t *tab) {
+    uint32_t r = 0;
+    int k = 1;
+    for (int p = 0; p < n; ++p) {
+        if ((sa >> p) & 1u) r += binom_sym(tab, 2 * p, k++);
+        if ((sb >> p) & 1u) r += binom_sym(tab, 2 * p + 1, k++);
+    }
+    return r;
+}
+
+// parity of the pairs (alpha a, beta b) with b < a: the blocked-to-interleaved sign
+QPE_HD uint32_t interleave_parity(uint32_t sa, uint32_t sb) {
+    uint32_t par = 0;
+    while (sa) {
+        const int a = __builtin_ctz(sa);
+        par += (uint32_t)popcnt(sb & ((1u << a) - 1u));
+        sa &= sa - 1;
+    }
+    return par & 1u;
+}
+
+// the alpha and beta strings of the interleaved string with colex rank r
+// (`ones` electrons on 2n spin-orbitals); false when the rank is out of range
+QPE_HD bool interleaved_unrank(uint32_t r, int ones, int n, const uint32_t *tab, uint32_t *sa, uint32_t *sb) {
+    uint32_t a = 0, b = 0;
+    int top = 2 * n - 1;
+    for (int i = ones; i >= 1; --i) {
+        while (top >= 0 && binom_sym(tab, top, i) > r) --top;
+        if (top < 0) return false;
+        if (top & 1) b |= 1u << (top >> 1);
+        else a |= 1u << (top >> 1);
+        r -= binom_sym(tab, top, i);
+        --top;
+    }
+    *sa = a;
+    *sb = b;
+    return r == 0;
+}
+
 // Pair k of a single excitation between orbitals x and y (discoverCA_BVs,
 // cpp/time_evolution_over_terms.hpp:1276-1324): the free string f has e-1
 // electrons on the other n-2 orbitals.  Returns rank(f|x) in *rx, rank(f|y) in

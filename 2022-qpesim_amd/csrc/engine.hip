@@ -1120,6 +1120,66 @@ __global__ void gather_kernel(const typename Vec<R>::type *state, double2 *stagi
     }
 }
 
+// Single-string register <-> alpha x beta block.  dir 0: ab[b*C + a] = sign *
+// k1[rank(interleave(a, b))]; dir 1: k1[...] = sign * ab[b*C + a].  One thread per
+// alpha x beta amplitude (column-major, alpha fastest): the reads (dir 0) or writes
+// (dir 1) on the single-string side are a permutation of the Sz = 0 entries.
+template <typename R>
+__global__ void __launch_bounds__(256) layout_kernel(typename Vec<R>::type *ab, typename Vec<R>::type *k1,
+                                                     const uint32_t *strings, const uint32_t *tab_g, uint32_t C, int n,
+                                                     int dir) {
+    __shared__ uint32_t tab[kBinomWords];
+    for (int i = threadIdx.x; i < kBinomWords; i += blockDim.x) tab[i] = tab_g[i];
+    __syncthreads();
+    const uint64_t D = (uint64_t)C * C;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < D; i += (uint64_t)gridDim.x * 256ull) {
+        const uint32_t b = (uint32_t)(i / C), a = (uint32_t)(i - (uint64_t)b * C);
+        const uint32_t sa = strings[a], sb = strings[b];
+        const uint32_t r = interleaved_rank(sa, sb, n, tab);
+        const bool neg = interleave_parity(sa, sb) != 0u;
+        auto v = dir == 0 ? k1[r] : ab[i];
+        if (neg) {
+            v.x = -v.x;
+            v.y = -v.y;
+        }
+        if (dir == 0) ab[i] = v;
+        else k1[r] = v;
+    }
+}
+
+// flags (sets *flag) a non-zero single-string amplitude outside the (e alpha, e beta) sector
+template <typename R>
+__global__ void __launch_bounds__(256) outside_sector_kernel(const typename Vec<R>::type *k1, uint64_t D1,
+                                                             const uint32_t *tab_g, int n, int e, int *flag) {
+    __shared__ uint32_t tab[kBinomWords];
+    for (int i = threadIdx.x; i < kBinomWords; i += blockDim.x) tab[i] = tab_g[i];
+    __syncthreads();
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < D1; i += (uint64_t)gridDim.x * 256ull) {
+        const auto v = k1[i];
+        if (v.x == 0 && v.y == 0) continue;
+        uint32_t sa = 0, sb = 0;
+        if (!interleaved_unrank((uint32_t)i, 2 * e, n, tab, &sa, &sb) || popcnt(sa) != e) atomicOr(flag, 1);
+    }
+}
+
+// single-string register <-> host staging (interleaved re, im doubles), with dtype conversion
+template <typename R>
+__global__ void copy_in_kernel(typename Vec<R>::type *state, const double2 *staging, uint64_t offset, uint64_t count) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256ull) {
+        const double2 v = staging[i];
+        auto &o = state[offset + i];
+        o.x = (R)v.x;
+        o.y = (R)v.y;
+    }
+}
+template <typename R>
+__global__ void copy_out_kernel(const typename Vec<R>::type *state, double2 *staging, uint64_t offset, uint64_t count) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256ull) {
+        const auto v = state[offset + i];
+        staging[i] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
 template <typename R>
 __global__ void scale_kernel(typename Vec<R>::type *blk, uint64_t D, double f) {
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < D; i += (uint64_t)gridDim.x * 256ull) {
@@ -1293,34 +1353,42 @@ static int norm_items_of(const Plan &p) {
     return k;
 }
 
+// one step with explicit block pointers: cur = U prev, block index j (norms[j], scale[j];
+// scale[j-1] renormalises prev; j also seeds the rounding dither)
+template <typename R>
+static void evolve_pointers_t(const DevicePlan &dp, void *prev_v, void *cur_v, uint64_t j, double *norms, double *scale,
+                              int renorm, int *status_flag, hipStream_t s) {
+    using V = typename Vec<R>::type;
+    const Plan &p = dp.plan;
+    const PassArgs A = make_args(dp, cur_v, scale, renorm);
+    V *prev = reinterpret_cast<V *>(prev_v), *cur = reinterpret_cast<V *>(cur_v);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    for (size_t k = 0; k < p.passes.size(); ++k) {
+        const PassDev &ps = p.passes[k];
+        if (ps.kind == PASS_COLG)
+            hipLaunchKernelGGL((pass_kernel<R, 2>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev,
+                               cur, j);
+        else
+            hipLaunchKernelGGL((pass_kernel<R, 1>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev,
+                               cur, j);
+    }
+    hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items_of(p), j, norms,
+                       scale, status_flag);
+    hip_check(hipGetLastError(), "pass launch");
+}
+
 template <typename R>
 static void evolve_t(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms,
                      double *scale, int renorm, int *status_flag, hipStream_t s, int *launches) {
     using V = typename Vec<R>::type;
     const Plan &p = dp.plan;
-    const PassArgs A = make_args(dp, state, scale, renorm);
-    const uint64_t D = A.D;
+    const uint64_t D = (uint64_t)p.C * p.C;
     V *base = reinterpret_cast<V *>(state);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
-    const int norm_items = norm_items_of(p);
-    for (uint64_t j = j_begin; j < j_end; ++j) {
-        V *prev = base + (j - 1) * D, *cur = base + j * D;
-        for (size_t k = 0; k < p.passes.size(); ++k) {
-            const PassDev &ps = p.passes[k];
-            if (ps.kind == PASS_COLG)
-                hipLaunchKernelGGL((pass_kernel<R, 2>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps,
-                                   prev, cur, j);
-            else
-                hipLaunchKernelGGL((pass_kernel<R, 1>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps,
-                                   prev, cur, j);
-        }
-        hipLaunchKernelGGL(norm_finalize, dim3(1), dim3(1024), 0, s, (const double *)dp.partials, norm_items, j, norms,
-                           scale, status_flag);
-    }
-    hip_check(hipGetLastError(), "pass launch");
+    for (uint64_t j = j_begin; j < j_end; ++j)
+        evolve_pointers_t<R>(dp, base + (j - 1) * D, base + j * D, j, norms, scale, renorm, status_flag, s);
     if (launches) *launches = (int)p.passes.size() + 1;
 }
 
@@ -1613,6 +1681,57 @@ void gather_out(int dtype, const void *state, double *staging, uint64_t offset, 
         hipLaunchKernelGGL(gather_kernel<float>, dim3(grid_for(count)), dim3(256), 0, s,
                            reinterpret_cast<const float2 *>(state), reinterpret_cast<double2 *>(staging), offset, count, C);
     hip_check(hipGetLastError(), "gather launch");
+}
+
+void single_string_copy(int dtype, void *state, double *staging, uint64_t offset, uint64_t count, bool to_state,
+                        hipStream_t s) {
+    if (dtype == 0) {
+        if (to_state)
+            hipLaunchKernelGGL(copy_in_kernel<double>, dim3(grid_for(count)), dim3(256), 0, s,
+                               reinterpret_cast<double2 *>(state), reinterpret_cast<const double2 *>(staging), offset, count);
+        else
+            hipLaunchKernelGGL(copy_out_kernel<double>, dim3(grid_for(count)), dim3(256), 0, s,
+                               reinterpret_cast<const double2 *>(state), reinterpret_cast<double2 *>(staging), offset, count);
+    } else {
+        if (to_state)
+            hipLaunchKernelGGL(copy_in_kernel<float>, dim3(grid_for(count)), dim3(256), 0, s,
+                               reinterpret_cast<float2 *>(state), reinterpret_cast<const double2 *>(staging), offset, count);
+        else
+            hipLaunchKernelGGL(copy_out_kernel<float>, dim3(grid_for(count)), dim3(256), 0, s,
+                               reinterpret_cast<const float2 *>(state), reinterpret_cast<double2 *>(staging), offset, count);
+    }
+    hip_check(hipGetLastError(), "single-string copy launch");
+}
+
+void single_string_convert(const DevicePlan &dp, void *ab, void *k1, bool to_ab, hipStream_t s) {
+    const Plan &p = dp.plan;
+    const uint64_t D = (uint64_t)p.C * p.C;
+    if (dp.dtype == 0)
+        hipLaunchKernelGGL(layout_kernel<double>, dim3(grid_for(D)), dim3(256), 0, s, reinterpret_cast<double2 *>(ab),
+                           reinterpret_cast<double2 *>(k1), (const uint32_t *)dp.strings, (const uint32_t *)dp.binom, p.C,
+                           p.n, to_ab ? 0 : 1);
+    else
+        hipLaunchKernelGGL(layout_kernel<float>, dim3(grid_for(D)), dim3(256), 0, s, reinterpret_cast<float2 *>(ab),
+                           reinterpret_cast<float2 *>(k1), (const uint32_t *)dp.strings, (const uint32_t *)dp.binom, p.C,
+                           p.n, to_ab ? 0 : 1);
+    hip_check(hipGetLastError(), "layout launch");
+}
+
+void single_string_outside(const DevicePlan &dp, const void *k1, uint64_t D1, int *flag, hipStream_t s) {
+    const Plan &p = dp.plan;
+    if (dp.dtype == 0)
+        hipLaunchKernelGGL(outside_sector_kernel<double>, dim3(grid_for(D1)), dim3(256), 0, s,
+                           reinterpret_cast<const double2 *>(k1), D1, (const uint32_t *)dp.binom, p.n, p.e, flag);
+    else
+        hipLaunchKernelGGL(outside_sector_kernel<float>, dim3(grid_for(D1)), dim3(256), 0, s,
+                           reinterpret_cast<const float2 *>(k1), D1, (const uint32_t *)dp.binom, p.n, p.e, flag);
+    hip_check(hipGetLastError(), "sector check launch");
+}
+
+void evolve_step(const DevicePlan &dp, void *prev, void *cur, uint64_t j, double *norms, double *scale, int renorm,
+                 int *status_flag, hipStream_t s) {
+    if (dp.dtype == 0) evolve_pointers_t<double>(dp, prev, cur, j, norms, scale, renorm, status_flag, s);
+    else evolve_pointers_t<float>(dp, prev, cur, j, norms, scale, renorm, status_flag, s);
 }
 
 void scale_block(int dtype, void *state, uint64_t j, uint64_t D, double factor, hipStream_t s) {

@@ -130,6 +130,19 @@ void gather_out(int dtype, const void *state, double *staging, uint64_t offset, 
                 uint32_t C, hipStream_t s);
 void scale_block(int dtype, void *state, uint64_t j, uint64_t D, double factor, hipStream_t s);
 
+// One step cur = U prev with explicit block pointers (block index j: norms[j], scale[j]).
+void evolve_step(const DevicePlan &dp, void *prev, void *cur, uint64_t j, double *norms, double *scale, int renorm,
+                 int *status_flag, hipStream_t s);
+
+// Single-string (interleaved Jordan-Wigner) register: host staging copies in its own
+// index order; conversion of one block to / from the alpha x beta layout of the plan
+// (C x C, column-major) with the blocked-to-interleaved sign; check for amplitudes
+// outside the (e alpha, e beta) sector (sets *flag).
+void single_string_copy(int dtype, void *state, double *staging, uint64_t offset, uint64_t count, bool to_state,
+                        hipStream_t s);
+void single_string_convert(const DevicePlan &dp, void *ab, void *k1, bool to_ab, hipStream_t s);
+void single_string_outside(const DevicePlan &dp, const void *k1, uint64_t D1, int *flag, hipStream_t s);
+
 // 2x2 gate g (row-major, interleaved re/im) on bit tbit of the block index of a
 // [2^m][D] register; cbit >= 0: only on blocks whose bit cbit is 1
 void apply_gate(int dtype, void *state, uint64_t D, int m, int tbit, int cbit, const double *g, hipStream_t s);

@@ -11,6 +11,7 @@ cpp/main.cc:99-134):
 There is no CPU fallback: if the library is missing this module raises.
 """
 import ctypes
+import itertools
 import math
 import os
 
@@ -20,6 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libqpesim.so")
 
 COMPLEX128, COMPLEX64 = 0, 1
+LAYOUT_ALPHA_BETA, LAYOUT_SINGLE_STRING = 0, 1
 PP, PQ, PQQP, PQQR, PQRS = range(5)
 OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES, OPT_TROTTER_ORDER, OPT_TERM_ORDER = 0, 1, 2, 3, 4
 ERR_ARG, ERR_OOM, ERR_HIP, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
@@ -36,6 +38,7 @@ EXPORTS = [
     "qpe_hamiltonian_ingest_broombridge", "qpe_hamiltonian_ingest_broombridge_text", "qpe_hamiltonian_initial_state",
     "qpe_hamiltonian_identity_term", "qpe_hamiltonian_get_sizes", "qpe_state_ingest_broombridge", "qpe_energy_estimate",
     "qpe_time_evolve_iqpe", "qpe_state_apply_gate", "qpe_state_apply_controlled_gate", "qpe_qift_noalloc",
+    "qpe_state_create_layout",
 ]
 BATCH_EAGER = 1
 
@@ -89,6 +92,7 @@ def load():
         "qpe_state_apply_gate": [vp, P(dbl), ctypes.c_uint],
         "qpe_state_apply_controlled_gate": [vp, P(dbl), ctypes.c_uint, ctypes.c_uint],
         "qpe_qift_noalloc": [vp, vp],
+        "qpe_state_create_layout": [P(vp), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ci, ci],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -234,15 +238,17 @@ class Hamiltonian:
 class StateVector:
     """qpe_state_t: the reference's StateVector (cpp/state_vector.hpp:16-42) resident in HBM."""
 
-    def __init__(self, precision, num_orbitals, num_occupied, dtype=COMPLEX128):
+    def __init__(self, precision, num_orbitals, num_occupied, dtype=COMPLEX128, layout=LAYOUT_ALPHA_BETA):
         self._s = ctypes.c_void_p()
-        _call("qpe_state_create", ctypes.byref(self._s), int(precision), int(num_orbitals), int(num_occupied), int(dtype))
+        _call("qpe_state_create_layout", ctypes.byref(self._s), int(precision), int(num_orbitals), int(num_occupied),
+              int(dtype), int(layout))
         self.m = int(precision)
         self.M = 1 << self.m
         self.n = int(num_orbitals)
         self.e = int(num_occupied)
         self.C = math.comb(self.n, self.e)
-        self.D = self.C * self.C
+        self.layout = layout
+        self.D = self.C * self.C if layout == LAYOUT_ALPHA_BETA else math.comb(2 * self.n, 2 * self.e)
         self.dtype = dtype
 
     @classmethod
@@ -254,6 +260,7 @@ class StateVector:
         sv.n = int(num_orbitals)
         sv.e = int(num_occupied)
         sv.C = math.comb(sv.n, sv.e)
+        sv.layout = LAYOUT_ALPHA_BETA
         sv.D = sv.C * sv.C
         sv.dtype = dtype
         return sv
@@ -535,6 +542,35 @@ def initial_state(n_spatial, n_occupied, seed=None, hf_weight=1.0):
         psi = hf_weight * psi + (1.0 - hf_weight) * r
         psi /= np.linalg.norm(psi)
     return psi
+
+
+def single_string_map(n_spatial, n_occupied):
+    """Where each alpha x beta amplitude lives in the single-string register.
+
+    Returns (index, sign), two arrays over the C(n,e)^2 determinants in the
+    reference's order (alpha rank * C + beta rank): the colex rank of the
+    interleaved string (bit 2p = alpha orbital p, bit 2p+1 = beta orbital p) among
+    the C(2n, 2e) strings, and the sign (-1)^(pairs alpha a, beta b with b < a)
+    that reorders the creation operators from alpha-then-beta to interleaved.
+    amp_single[index] = sign * amp_alpha_beta."""
+    n, e = int(n_spatial), int(n_occupied)
+    strings = np.array(sorted(sum(1 << b for b in c) for c in itertools.combinations(range(n), e)), np.int64)
+    C = len(strings)
+    sa = np.repeat(strings, C)
+    sb = np.tile(strings, C)
+    index = np.zeros(C * C, np.int64)
+    k = np.zeros(C * C, np.int64)    # electrons placed so far (the 1-based order of the next one)
+    par = np.zeros(C * C, np.int64)
+    binom = np.array([[math.comb(i, j) for j in range(2 * n + 1)] for i in range(2 * n)], np.int64)
+    beta_below = np.zeros(C * C, np.int64)   # beta electrons on orbitals below p
+    for p in range(n):
+        for spin, s in ((0, sa), (1, sb)):
+            occ = (s >> p) & 1
+            k += occ
+            index += np.where(occ == 1, binom[2 * p + spin, k], 0)
+        par += ((sa >> p) & 1) * beta_below   # alpha p moves past every beta below it
+        beta_below += (sb >> p) & 1
+    return index, np.where(par & 1, -1.0, 1.0)
 
 
 # the workloads, by name: spatial orbitals, electrons per spin, ancillas, seed, HF weight

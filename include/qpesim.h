@@ -40,6 +40,13 @@ enum qpe_dtype {
     QPE_COMPLEX64 = 1     /* FP32 state and rotations; FP64 angles, norms, P(k) */
 };
 
+/* register layouts */
+enum qpe_layout {
+    QPE_LAYOUT_ALPHA_BETA = 0,    /* the reference's [2^m][C(n,e)][C(n,e)] product space (default) */
+    QPE_LAYOUT_SINGLE_STRING = 1  /* [2^m][C(2n,2e)]: one string of 2n spin-orbitals, interleaved
+                                     Jordan-Wigner order (no counterpart in the reference) */
+};
+
 /* term classes, in the order PrintIndexedSegmentedHamiltonianTerms returns them */
 enum qpe_term_class { QPE_PP = 0, QPE_PQ = 1, QPE_PQQP = 2, QPE_PQQR = 3, QPE_PQRS = 4 };
 
@@ -122,7 +129,22 @@ int qpe_state_create(qpe_state_t *out, unsigned precision, unsigned num_orbitals
                      unsigned num_occupied, int dtype);
 int qpe_state_destroy(qpe_state_t s);
 
-/* number of amplitudes 2^m * C(n,e)^2 */
+/* No counterpart in the reference, whose StateVector is the alpha x beta product
+ * space: a register in either layout.  QPE_LAYOUT_SINGLE_STRING holds
+ * [2^precision][C(2n, 2e)] amplitudes; spin-orbital 2p is alpha orbital p and
+ * 2p+1 beta orbital p; determinant x (a bit string of 2n bits with 2e set) is
+ * stored at its colex rank; qpe_state_set/get use that index order.  Its amplitude
+ * is sign * the alpha x beta amplitude of the same determinant, sign = (-1)^(number
+ * of pairs alpha a, beta b with b < a) (the reordering of the creation operators).
+ * The evolution works on the (e alpha, e beta) sector: each step converts block j-1
+ * to the alpha x beta layout, applies U there and writes block j back (amplitudes
+ * of other sectors in block 0 are rejected, QPE_ERR_ARG); QIFT, P(k), gates and
+ * scaling act on the single-string register directly.  Up to 17 spatial orbitals;
+ * time_evolve_iqpe and the batch need the alpha x beta layout. */
+int qpe_state_create_layout(qpe_state_t *out, unsigned precision, unsigned num_orbitals, unsigned num_occupied,
+                            int dtype, int layout);
+
+/* number of amplitudes 2^m * C(n,e)^2 (single-string layout: 2^m * C(2n,2e)) */
 int qpe_state_size(qpe_state_t s, uint64_t *n_amplitudes);
 
 /* StateVector::operator[] writes/reads, batched: amplitudes [offset, offset+count)

@@ -25,6 +25,19 @@ def sector_basis(n, e):
     return strings, rank, basis, index, C
 
 
+def interleaved_basis(n, e):
+    """The single-string register's basis: every string of 2e electrons on 2n
+    interleaved modes (alpha orbital p = mode 2p, beta orbital p = mode 2p+1),
+    including the Sz != 0 ones, in colex order (ascending bit mask)."""
+    basis = sorted(sum(1 << b for b in c) for c in itertools.combinations(range(2 * n), 2 * e))
+    return basis, {x: i for i, x in enumerate(basis)}
+
+
+def interleaved_mode(mode, n):
+    """blocked mode (alpha p = p, beta p = n + p) -> interleaved mode"""
+    return 2 * mode if mode < n else 2 * (mode - n) + 1
+
+
 def apply_ops(ops, x):
     """ops: list of (mode, dagger) applied right-to-left; returns (sign, x') or (0, None)."""
     sign = 1
@@ -39,9 +52,15 @@ def apply_ops(ops, x):
     return sign, x
 
 
-def operator_matrix(terms, n, e):
-    """sum_k c_k * product_k on the sector; terms = [(c, [(mode, dagger), ...])]."""
-    _, _, basis, index, _ = sector_basis(n, e)
+def operator_matrix(terms, n, e, interleaved=False):
+    """sum_k c_k * product_k on the sector; terms = [(c, [(mode, dagger), ...])]
+    with blocked mode numbers.  interleaved: the same operators with the modes
+    renumbered to the interleaved order, on the single-string basis."""
+    if interleaved:
+        basis, index = interleaved_basis(n, e)
+        terms = [(c, [(interleaved_mode(md, n), dg) for md, dg in ops]) for c, ops in terms]
+    else:
+        _, _, basis, index, _ = sector_basis(n, e)
     dim = len(basis)
     Mx = np.zeros((dim, dim), dtype=np.complex128)
     for c, ops in terms:
@@ -52,7 +71,7 @@ def operator_matrix(terms, n, e):
     return Mx
 
 
-def term_generator(cls, idx, coeff, n, e):
+def term_generator(cls, idx, coeff, n, e, interleaved=False):
     """Hermitian generator of one indexed reference term."""
     cr = lambda m: (m, True)
     an = lambda m: (m, False)
@@ -68,10 +87,10 @@ def term_generator(cls, idx, coeff, n, e):
     else:             # pqqr / pqrs: c (a+_p a+_q a_r a_s + h.c.)
         p, q, r, s = idx
         ops = [(coeff, [cr(p), cr(q), an(r), an(s)]), (coeff, [cr(s), cr(r), an(q), an(p)])]
-    return operator_matrix(ops, n, e)
+    return operator_matrix(ops, n, e, interleaved)
 
 
-def textbook_hamiltonian(n, e, one_body, two_body):
+def textbook_hamiltonian(n, e, one_body, two_body, interleaved=False):
     """H = sum h_pq a+_{p s} a_{q s} + 1/2 sum (pq|rs) a+_{p s} a+_{r s'} a_{s s'} a_{q s}
     from Broombridge-style lists (1-based, Mulliken two-body, each 8-fold class once)."""
     h = np.zeros((n, n))
@@ -97,4 +116,4 @@ def textbook_hamiltonian(n, e, one_body, two_body):
                 if v == 0.0:
                     continue
                 terms.append((0.5 * v, [(s1 * n + p, True), (s2 * n + r, True), (s2 * n + s, False), (s1 * n + q, False)]))
-    return operator_matrix(terms, n, e)
+    return operator_matrix(terms, n, e, interleaved)
