@@ -61,7 +61,10 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     P.strings.resize(C);
     for (uint32_t i = 0; i < C; ++i) P.strings[i] = unrank_of(i, e, n, B.tab);
 
-    if (C > kSchedPad) throw std::runtime_error("C(num_orbitals, num_occupied) exceeds the 14-bit rank range of schedule entries");
+    if (C > kSchedPad)
+        throw std::runtime_error("C(num_orbitals, num_occupied) = " + std::to_string(C) + " exceeds " +
+                                 std::to_string(kSchedPad) +
+                                 ", the largest string count the 14-bit ranks of the pass schedules hold");
     // single-excitation pair tables for every ordered orbital pair (x, y)
     P.ca_table.assign((size_t)n * n * P.ca_pairs, 0u);
     for (int x = 0; x < n; ++x)

@@ -182,7 +182,10 @@ int qpe_state_ingest_broombridge(qpe_state_t *out, qpe_hamiltonian_t h, const ch
 
 /* time_evolve_over_terms(sv, h, m), cpp/time_evolution_over_terms.hpp:92-139:
  * block j = U * block(j-1) for j = 1..2^m-1, U one first-order Trotter step in
- * the reference's term order. */
+ * the reference's term order.  Limit: C(n, e) <= 16383 (the pass schedules hold
+ * 14-bit string ranks; C(17,5) = 6188 is the largest workload here); larger
+ * states can be created, set, transformed and read, but evolving them returns
+ * QPE_ERR_ARG. */
 int qpe_time_evolve_over_terms(qpe_state_t s, qpe_hamiltonian_t h);
 
 /* time_evolve_over_terms restricted to blocks j_begin .. j_end-1 (1 <= j_begin

@@ -81,6 +81,15 @@ def test_plan_small_tiles_still_valid():
     assert ei.value.code == qpesim.ERR_ARG
 
 
+def test_string_count_limit_of_the_pass_schedules():
+    """Schedule entries hold 14-bit ranks: C(18,6) = 18564 cannot be planned."""
+    one, two = qpesim.synthetic_integrals(18, 0)
+    h, _ = qpesim.build_hamiltonian(18, 6, one, two)
+    with pytest.raises(qpesim.QpeError) as ei:
+        qpesim.plan_info(h, qpesim.COMPLEX64, 0)
+    assert ei.value.code == qpesim.ERR_ARG and "16383" in str(ei.value)
+
+
 def test_argument_errors_without_gpu():
     h = qpesim.Hamiltonian()
     with pytest.raises(qpesim.QpeError) as ei:
