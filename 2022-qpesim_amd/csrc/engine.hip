@@ -15,6 +15,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "combinadic.hpp"
@@ -1079,6 +1080,211 @@ qift_kernel(typename Vec<R>::type *state, uint64_t D, int m, int w, const double
     }
 }
 
+// ---- QIFT, register form (2^m <= 1024) ------------------------------------------
+// Y[k] = 2^{-m/2} sum_j scale[j] X[j] e^{-2 pi i jk/M} per column, as two rounds of
+// in-register DFTs (the "four-step" split M = R1 * R2, j = j1 + R1 j2, k = k2 + R2 k1):
+//   A  thread (j1, column) loads X[j1 + R1 j2] for all j2 straight from HBM, takes the
+//      R2-point DFT over j2 in registers and multiplies by e^{-2 pi i j1 k2 / M};
+//   C  thread (k2, column) reads the R1 values Z[k2][j1] through LDS, takes the R1-point
+//      DFT over j1 and writes Y[k2 + R2 k1] back to HBM.
+// LDS holds one exchange of the tile (in the state's type) instead of log2(M) radix-2
+// stages.  A tile is W columns: W * sizeof(amplitude) = 128 bytes or more per row.
+template <typename R, int LOGM> struct QiftShape {
+    static constexpr int M = 1 << LOGM;
+    static constexpr int LR1 = LOGM / 2, LR2 = LOGM - LR1;
+    static constexpr int R1 = 1 << LR1, R2 = 1 << LR2;
+    static constexpr int S = (int)sizeof(typename Vec<R>::type);
+    // 128-byte rows, except at 2^m = 1024, where 32-point DFTs on 512 threads would
+    // spill: 256 threads (8 columns, 64-byte rows of complex64) keep them in registers
+    static constexpr int W0 = R1 >= 32 ? 256 / R2 : 256 / R2 > 128 / S ? 256 / R2 : 128 / S;
+    static constexpr int W = W0 > 64 ? 64 : W0;    // the lanes of one column group share a wave
+    static constexpr int T = R2 * W;                // threads (step C: all; step A: R1 * W)
+    static constexpr int PAD = 128 / S;             // row pad: the next k2 row starts 128 bytes of banks later
+    static constexpr int PITCH = R1 * W + PAD;      // elements per k2 row of the exchange tile
+    static constexpr int OWN = R1 / W > 1 ? R1 / W : 1;   // P(k) values each lane keeps after the reduction
+    static constexpr int LW = W >= 64 ? 6 : W >= 32 ? 5 : W >= 16 ? 4 : W >= 8 ? 3 : W >= 4 ? 2 : W >= 2 ? 1 : 0;
+    static constexpr size_t lds_bytes() {
+        return (size_t)(M / 2 > 0 ? M / 2 : 1) * sizeof(double2) + (size_t)M * sizeof(double) + (size_t)R2 * PITCH * S;
+    }
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// f(integral_constant<int, I>) for I = 0 .. N-1: array indices that are constants from
+// the start, so the arrays below live in registers (a runtime loop index, even one the
+// compiler unrolls later, can leave an array in memory behind selects of addresses)
+template <int I, int N, class F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+template <int L> constexpr int rev_bits(int x) {
+    int r = 0;
+    for (int i = 0; i < L; ++i) r |= ((x >> i) & 1) << (L - 1 - i);
+    return r;
+}
+
+// one radix-2 stage of span `len` (DIT, bit-reversed input order)
+template <int N, int LEN>
+__device__ __forceinline__ void dft_stage(double2 (&a)[N], const double2 *tw, int M) {
+    constexpr int H = LEN / 2;
+    static_for<0, H>([&](auto kk) {
+        const double2 z = kk.value ? tw[kk.value * (M / LEN)] : make_double2(1.0, 0.0);
+        static_for<0, N / LEN>([&](auto g) {
+            constexpr int i0 = g.value * LEN + kk.value, i1 = i0 + H;
+            const double2 u = a[i0];
+            const double2 t = kk.value ? cmul(a[i1], z) : a[i1];
+            a[i0] = make_double2(u.x + t.x, u.y + t.y);
+            a[i1] = make_double2(u.x - t.x, u.y - t.y);
+        });
+        // one twiddle in flight at a time: the scheduler would otherwise load every
+        // stage's twiddles up front and run out of registers
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+// forward DFT of 2^L points in registers, natural order in and out;
+// tw[k] = e^{-2 pi i k / M} for k < M / 2
+template <int L>
+__device__ __forceinline__ void reg_dft(double2 (&v)[1 << L], const double2 *tw, int M) {
+    constexpr int N = 1 << L;
+    if constexpr (L > 0) {
+        double2 a[N];
+        static_for<0, N>([&](auto i) { a[rev_bits<L>(i.value)] = v[i.value]; });
+        static_for<1, L + 1>([&](auto st) { dft_stage<N, (1 << st.value)>(a, tw, M); });
+        static_for<0, N>([&](auto i) { v[i.value] = a[i.value]; });
+    }
+}
+
+template <typename R, int LOGM>
+__global__ void __launch_bounds__((QiftShape<R, LOGM>::T))
+qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, const double2 *twiddles,
+                double *pk_partial) {
+    using Q = QiftShape<R, LOGM>;
+    using V = typename Vec<R>::type;
+    constexpr int M = Q::M, R1 = Q::R1, R2 = Q::R2, W = Q::W;
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2 *tw = reinterpret_cast<double2 *>(smem);                 // [max(1, M/2)]
+    double *sc = reinterpret_cast<double *>(tw + (M / 2 > 0 ? M / 2 : 1));   // [M]
+    V *Z = reinterpret_cast<V *>(sc + M);                            // [R2][PITCH]
+    for (int i = threadIdx.x; i < M / 2; i += Q::T) tw[i] = twiddles[i];
+    for (int i = threadIdx.x; i < M; i += Q::T) sc[i] = scale[i];
+    const double inv = 1.0 / sqrt((double)M);
+    double acc[Q::OWN], cmp[Q::OWN];
+    static_for<0, Q::OWN>([&](auto i) { acc[i.value] = 0.0; cmp[i.value] = 0.0; });
+    // tiles in an order that gives workgroups on one XCD (blockIdx mod 8) neighbouring columns
+    const uint64_t ntiles = (D + W - 1) / W;
+    const uint32_t G = gridDim.x;
+    const bool by_xcd = G % 8 == 0;   // the host sizes the grid to a multiple of 8 when it can
+    const uint32_t rank = by_xcd ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+    __syncthreads();
+    for (uint64_t t = rank; t < ntiles; t += G) {
+        // opaque copies of D and of the thread index per tile keep the compiler from
+        // hoisting every row's offset, scale factor and twiddle address out of the loop
+        // (they would hold hundreds of registers)
+        uint64_t Dt = D;
+        int tid = threadIdx.x;
+        asm volatile("" : "+s"(Dt), "+v"(tid)::"memory");
+        const int c = tid % W;          // column inside the tile (both steps)
+        const int row = tid / W;        // step A: j1 (active when < R1); step C: k2
+        const uint64_t d0 = t * (uint64_t)W;
+        const int cols = (int)((Dt - d0) < (uint64_t)W ? (Dt - d0) : (uint64_t)W);
+        const bool live = c < cols;
+        if (row < R1) {
+            const int j1 = row;
+            V x[R2];
+            const V *src = state + (uint64_t)j1 * Dt + d0 + (live ? c : 0);
+            static_for<0, R2>([&](auto q) { x[q.value] = src[(uint64_t)(R1 * q.value) * Dt]; });
+            double2 v[R2];
+            static_for<0, R2>([&](auto q) {
+                const double f = live ? sc[j1 + R1 * q.value] : 0.0;
+                v[q.value] = make_double2((double)x[q.value].x * f, (double)x[q.value].y * f);
+            });
+            reg_dft<Q::LR2>(v, tw, M);
+            static_for<1, R2>([&](auto q) {
+                const int e = j1 * q.value;   // < M
+                double2 z = tw[e & (M / 2 - 1)];
+                if (e >= M / 2) z = make_double2(-z.x, -z.y);
+                v[q.value] = cmul(v[q.value], z);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            static_for<0, R2>([&](auto q) {
+                V o;
+                o.x = (R)v[q.value].x;
+                o.y = (R)v[q.value].y;
+                Z[q.value * Q::PITCH + j1 * W + c] = o;
+            });
+        }
+        __syncthreads();
+        const int k2 = row;
+        double2 u[R1];
+        static_for<0, R1>([&](auto q) {
+            const V z = Z[k2 * Q::PITCH + q.value * W + c];
+            u[q.value] = make_double2((double)z.x, (double)z.y);
+        });
+        reg_dft<Q::LR1>(u, tw, M);
+        double p[R1];
+        V *dst = state + (uint64_t)k2 * Dt + d0 + c;
+        static_for<0, R1>([&](auto q) {
+            const double2 y = make_double2(u[q.value].x * inv, u[q.value].y * inv);
+            p[q.value] = live ? y.x * y.x + y.y * y.y : 0.0;
+            if (live) {
+                V o;
+                o.x = (R)y.x;
+                o.y = (R)y.y;
+                dst[(uint64_t)(R2 * q.value) * Dt] = o;
+            }
+        });
+        // sum over the W columns of the tile: a halving exchange across the lanes of the
+        // column group leaves each lane OWN of the R1 sums (every lane all of them when R1 < W)
+        static_for<0, Q::LW>([&](auto st) {
+            constexpr int mask = 1 << st.value;
+            constexpr int n = R1 >> st.value;      // values still held before this step
+            const bool hi = (c & mask) != 0;
+            if constexpr (n > 1) {
+                constexpr int h = n / 2;
+                static_for<0, h>([&](auto i) {
+                    const double lo_v = p[i.value], hi_v = p[h + i.value];
+                    const double send = hi ? lo_v : hi_v;
+                    const double keep = hi ? hi_v : lo_v;
+                    p[i.value] = keep + __shfl_xor(send, mask, W);
+                });
+            } else {
+                p[0] += __shfl_xor(p[0], mask, W);
+            }
+        });
+        static_for<0, Q::OWN>([&](auto i) {
+            const double x = p[i.value], a = acc[i.value];
+            const double s2 = a + x;
+            if (fabs(a) >= fabs(x)) cmp[i.value] += (a - s2) + x;
+            else cmp[i.value] += (x - s2) + a;
+            acc[i.value] = s2;
+        });
+        __syncthreads();
+    }
+    // the k1 values a lane holds: bit b of its column picks half b of what remained
+    const int c = threadIdx.x % W, k2 = threadIdx.x / W;
+    int base = 0, n = R1;
+    bool writer = true;   // one lane of each group holding the same sums writes them
+    for (int mask = 1; mask < W; mask <<= 1) {
+        if (n > 1) {
+            n >>= 1;
+            if (c & mask) base += n;
+        } else if (c & mask) {
+            writer = false;
+        }
+    }
+    if (writer) {
+        static_for<0, Q::OWN>([&](auto i) {
+            pk_partial[(uint64_t)blockIdx.x * M + (uint32_t)(k2 + R2 * (base + i.value))] = acc[i.value] + cmp[i.value];
+        });
+    }
+}
+
 __global__ void pk_reduce(const double *pk_partial, int grid, uint32_t M, double *probs) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= M) return;
@@ -1578,8 +1784,45 @@ void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, d
     hip_check(hipGetLastError(), "block norm launch");
 }
 
+template <typename R, int LOGM>
+static bool launch_qift_reg(void *state, uint64_t D, const double *scale, const double *twiddles, double *pk_partial,
+                            int max_grid, size_t max_lds, int *grid_out, hipStream_t s) {
+    using Q = QiftShape<R, LOGM>;
+    const size_t lds = Q::lds_bytes();
+    if (lds > max_lds) return false;
+    const uint64_t ntiles = (D + Q::W - 1) / Q::W;
+    int grid = (int)std::min<uint64_t>(ntiles, (uint64_t)std::min(1024, max_grid));
+    if (grid >= 8) grid &= ~7;
+    if (grid < 1) throw std::runtime_error("no workgroup fits the P(k) partial sums");
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&qift_reg_kernel<R, LOGM>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((qift_reg_kernel<R, LOGM>), dim3(grid), dim3(Q::T), lds, s,
+                       reinterpret_cast<typename Vec<R>::type *>(state), D, scale,
+                       reinterpret_cast<const double2 *>(twiddles), pk_partial);
+    hip_check(hipGetLastError(), "qift launch");
+    *grid_out = grid;
+    return true;
+}
+
+template <typename R>
+static bool qift_reg(int m, void *state, uint64_t D, const double *scale, const double *twiddles, double *pk_partial,
+                     int max_grid, size_t max_lds, int *grid_out, hipStream_t s) {
+    switch (m) {
+#define QPE_QIFT_REG(L) \
+    case L: return launch_qift_reg<R, L>(state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s);
+        QPE_QIFT_REG(1) QPE_QIFT_REG(2) QPE_QIFT_REG(3) QPE_QIFT_REG(4) QPE_QIFT_REG(5)
+        QPE_QIFT_REG(6) QPE_QIFT_REG(7) QPE_QIFT_REG(8) QPE_QIFT_REG(9) QPE_QIFT_REG(10)
+#undef QPE_QIFT_REG
+    default: return false;
+    }
+}
+
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial, int *grid_out,
           double *twiddles, size_t max_lds, hipStream_t s, int max_grid) {
+    // 2^m <= 1024: the register form; larger: radix-2 stages in LDS
+    if (dtype == 0 ? qift_reg<double>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
+                   : qift_reg<float>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s))
+        return;
     const uint32_t M = 1u << m;
     const size_t tw_bytes = (size_t)std::max<uint32_t>(1, M / 2) * sizeof(double2);
     int w = 1;
