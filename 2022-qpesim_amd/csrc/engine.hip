@@ -1087,30 +1087,38 @@ qift_kernel(typename Vec<R>::type *state, uint64_t D, int m, int w, const double
 //      R2-point DFT over j2 in registers and multiplies by e^{-2 pi i j1 k2 / M};
 //   C  thread (k2, column) reads the R1 values Z[k2][j1] through LDS, takes the R1-point
 //      DFT over j1 and writes Y[k2 + R2 k1] back to HBM.
-// LDS holds one exchange of the tile (in the state's type) instead of log2(M) radix-2
-// stages.  A tile is W columns: W * sizeof(amplitude) = 128 bytes or more per row.
+// LDS holds one exchange of the tile instead of log2(M) radix-2 stages.  A tile is W
+// columns: W * sizeof(amplitude) = 128 bytes or more per row.  The DFTs run in the
+// state's precision (FP32 for complex64, like the rotations); |Y|^2 and the P(k) sums
+// are FP64 with compensation.
 template <typename R, int LOGM> struct QiftShape {
     static constexpr int M = 1 << LOGM;
     static constexpr int LR1 = LOGM / 2, LR2 = LOGM - LR1;
     static constexpr int R1 = 1 << LR1, R2 = 1 << LR2;
     static constexpr int S = (int)sizeof(typename Vec<R>::type);
-    // 128-byte rows, except at 2^m = 1024, where 32-point DFTs on 512 threads would
-    // spill: 256 threads (8 columns, 64-byte rows of complex64) keep them in registers
-    static constexpr int W0 = R1 >= 32 ? 256 / R2 : 256 / R2 > 128 / S ? 256 / R2 : 128 / S;
+    static constexpr int W0 = 256 / R2 > 128 / S ? 256 / R2 : 128 / S;   // rows of 128 bytes or more
     static constexpr int W = W0 > 64 ? 64 : W0;    // the lanes of one column group share a wave
     static constexpr int T = R2 * W;                // threads (step C: all; step A: R1 * W)
     static constexpr int PAD = 128 / S;             // row pad: the next k2 row starts 128 bytes of banks later
     static constexpr int PITCH = R1 * W + PAD;      // elements per k2 row of the exchange tile
     static constexpr int OWN = R1 / W > 1 ? R1 / W : 1;   // P(k) values each lane keeps after the reduction
+    // loads one tile ahead: measured faster for complex128 (0.49 against 0.42 of the HBM
+    // peak at 2^m = 1024); complex64 at 2^m = 1024 would spill (0.32 against 0.47)
+    static constexpr bool PREFETCH = S == 16;
     static constexpr int LW = W >= 64 ? 6 : W >= 32 ? 5 : W >= 16 ? 4 : W >= 8 ? 3 : W >= 4 ? 2 : W >= 2 ? 1 : 0;
     static constexpr size_t lds_bytes() {
-        return (size_t)(M / 2 > 0 ? M / 2 : 1) * sizeof(double2) + (size_t)M * sizeof(double) + (size_t)R2 * PITCH * S;
+        return (size_t)M * sizeof(double) + ((size_t)(M / 2 > 0 ? M / 2 : 1) + (size_t)R2 * PITCH) * S;
     }
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C2> __device__ __forceinline__ C2 cadd(C2 a, C2 b) { C2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C2> __device__ __forceinline__ C2 csub(C2 a, C2 b) { C2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
 
 // f(integral_constant<int, I>) for I = 0 .. N-1: array indices that are constants from
 // the start, so the arrays below live in registers (a runtime loop index, even one the
@@ -1128,18 +1136,18 @@ template <int L> constexpr int rev_bits(int x) {
     return r;
 }
 
-// one radix-2 stage of span `len` (DIT, bit-reversed input order)
-template <int N, int LEN>
-__device__ __forceinline__ void dft_stage(double2 (&a)[N], const double2 *tw, int M) {
+// one radix-2 stage of span LEN (DIT, bit-reversed input order)
+template <int N, int LEN, typename C2>
+__device__ __forceinline__ void dft_stage(C2 (&a)[N], const C2 *tw, int M) {
     constexpr int H = LEN / 2;
     static_for<0, H>([&](auto kk) {
-        const double2 z = kk.value ? tw[kk.value * (M / LEN)] : make_double2(1.0, 0.0);
+        const C2 z = tw[kk.value * (M / LEN)];
         static_for<0, N / LEN>([&](auto g) {
             constexpr int i0 = g.value * LEN + kk.value, i1 = i0 + H;
-            const double2 u = a[i0];
-            const double2 t = kk.value ? cmul(a[i1], z) : a[i1];
-            a[i0] = make_double2(u.x + t.x, u.y + t.y);
-            a[i1] = make_double2(u.x - t.x, u.y - t.y);
+            const C2 u = a[i0];
+            const C2 t = kk.value ? cmul(a[i1], z) : a[i1];
+            a[i0] = cadd(u, t);
+            a[i1] = csub(u, t);
         });
         // one twiddle in flight at a time: the scheduler would otherwise load every
         // stage's twiddles up front and run out of registers
@@ -1149,11 +1157,11 @@ __device__ __forceinline__ void dft_stage(double2 (&a)[N], const double2 *tw, in
 
 // forward DFT of 2^L points in registers, natural order in and out;
 // tw[k] = e^{-2 pi i k / M} for k < M / 2
-template <int L>
-__device__ __forceinline__ void reg_dft(double2 (&v)[1 << L], const double2 *tw, int M) {
+template <int L, typename C2>
+__device__ __forceinline__ void reg_dft(C2 (&v)[1 << L], const C2 *tw, int M) {
     constexpr int N = 1 << L;
     if constexpr (L > 0) {
-        double2 a[N];
+        C2 a[N];
         static_for<0, N>([&](auto i) { a[rev_bits<L>(i.value)] = v[i.value]; });
         static_for<1, L + 1>([&](auto st) { dft_stage<N, (1 << st.value)>(a, tw, M); });
         static_for<0, N>([&](auto i) { v[i.value] = a[i.value]; });
@@ -1168,12 +1176,17 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
     using V = typename Vec<R>::type;
     constexpr int M = Q::M, R1 = Q::R1, R2 = Q::R2, W = Q::W;
     extern __shared__ __align__(16) unsigned char smem[];
-    double2 *tw = reinterpret_cast<double2 *>(smem);                 // [max(1, M/2)]
-    double *sc = reinterpret_cast<double *>(tw + (M / 2 > 0 ? M / 2 : 1));   // [M]
-    V *Z = reinterpret_cast<V *>(sc + M);                            // [R2][PITCH]
-    for (int i = threadIdx.x; i < M / 2; i += Q::T) tw[i] = twiddles[i];
+    double *sc = reinterpret_cast<double *>(smem);                     // [M]
+    V *tw = reinterpret_cast<V *>(sc + M);                             // [max(1, M/2)]
+    V *Z = tw + (M / 2 > 0 ? M / 2 : 1);                               // [R2][PITCH]
+    for (int i = threadIdx.x; i < M / 2; i += Q::T) {
+        V z;
+        z.x = (R)twiddles[i].x;
+        z.y = (R)twiddles[i].y;
+        tw[i] = z;
+    }
     for (int i = threadIdx.x; i < M; i += Q::T) sc[i] = scale[i];
-    const double inv = 1.0 / sqrt((double)M);
+    const R inv = (R)(1.0 / sqrt((double)M));
     double acc[Q::OWN], cmp[Q::OWN];
     static_for<0, Q::OWN>([&](auto i) { acc[i.value] = 0.0; cmp[i.value] = 0.0; });
     // tiles in an order that gives workgroups on one XCD (blockIdx mod 8) neighbouring columns
@@ -1181,6 +1194,16 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
     const uint32_t G = gridDim.x;
     const bool by_xcd = G % 8 == 0;   // the host sizes the grid to a multiple of 8 when it can
     const uint32_t rank = by_xcd ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+    // step A's raw row values; with PREFETCH they are loaded one tile ahead, so that the
+    // loads of tile t + G are in flight while tile t finishes
+    V x[R2];
+    auto load_rows = [&](uint64_t tt, uint64_t Dt, int c, int j1) {
+        const uint64_t d0 = tt * (uint64_t)W;
+        const bool live = (uint64_t)c < Dt - d0;
+        const V *src = state + (uint64_t)j1 * Dt + d0 + (live ? c : 0);
+        static_for<0, R2>([&](auto q) { x[q.value] = src[(uint64_t)(R1 * q.value) * Dt]; });
+    };
+    if (Q::PREFETCH && rank < ntiles && (int)threadIdx.x / W < R1) load_rows(rank, D, threadIdx.x % W, threadIdx.x / W);
     __syncthreads();
     for (uint64_t t = rank; t < ntiles; t += G) {
         // opaque copies of D and of the thread index per tile keep the compiler from
@@ -1196,48 +1219,40 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
         const bool live = c < cols;
         if (row < R1) {
             const int j1 = row;
-            V x[R2];
-            const V *src = state + (uint64_t)j1 * Dt + d0 + (live ? c : 0);
-            static_for<0, R2>([&](auto q) { x[q.value] = src[(uint64_t)(R1 * q.value) * Dt]; });
-            double2 v[R2];
+            if (!Q::PREFETCH) load_rows(t, Dt, c, j1);
+            V v[R2];
             static_for<0, R2>([&](auto q) {
-                const double f = live ? sc[j1 + R1 * q.value] : 0.0;
-                v[q.value] = make_double2((double)x[q.value].x * f, (double)x[q.value].y * f);
+                const R f = live ? (R)sc[j1 + R1 * q.value] : (R)0;
+                v[q.value].x = x[q.value].x * f;
+                v[q.value].y = x[q.value].y * f;
             });
             reg_dft<Q::LR2>(v, tw, M);
             static_for<1, R2>([&](auto q) {
                 const int e = j1 * q.value;   // < M
-                double2 z = tw[e & (M / 2 - 1)];
-                if (e >= M / 2) z = make_double2(-z.x, -z.y);
+                V z = tw[e & (M / 2 - 1)];
+                if (e >= M / 2) {
+                    z.x = -z.x;
+                    z.y = -z.y;
+                }
                 v[q.value] = cmul(v[q.value], z);
                 __builtin_amdgcn_sched_barrier(0);
             });
-            static_for<0, R2>([&](auto q) {
-                V o;
-                o.x = (R)v[q.value].x;
-                o.y = (R)v[q.value].y;
-                Z[q.value * Q::PITCH + j1 * W + c] = o;
-            });
+            static_for<0, R2>([&](auto q) { Z[q.value * Q::PITCH + j1 * W + c] = v[q.value]; });
+            if (Q::PREFETCH && t + G < ntiles) load_rows(t + G, Dt, c, j1);
         }
         __syncthreads();
         const int k2 = row;
-        double2 u[R1];
-        static_for<0, R1>([&](auto q) {
-            const V z = Z[k2 * Q::PITCH + q.value * W + c];
-            u[q.value] = make_double2((double)z.x, (double)z.y);
-        });
+        V u[R1];
+        static_for<0, R1>([&](auto q) { u[q.value] = Z[k2 * Q::PITCH + q.value * W + c]; });
         reg_dft<Q::LR1>(u, tw, M);
         double p[R1];
         V *dst = state + (uint64_t)k2 * Dt + d0 + c;
         static_for<0, R1>([&](auto q) {
-            const double2 y = make_double2(u[q.value].x * inv, u[q.value].y * inv);
-            p[q.value] = live ? y.x * y.x + y.y * y.y : 0.0;
-            if (live) {
-                V o;
-                o.x = (R)y.x;
-                o.y = (R)y.y;
-                dst[(uint64_t)(R2 * q.value) * Dt] = o;
-            }
+            V y;
+            y.x = u[q.value].x * inv;
+            y.y = u[q.value].y * inv;
+            p[q.value] = live ? (double)y.x * (double)y.x + (double)y.y * (double)y.y : 0.0;
+            if (live) dst[(uint64_t)(R2 * q.value) * Dt] = y;
         });
         // sum over the W columns of the tile: a halving exchange across the lanes of the
         // column group leaves each lane OWN of the R1 sums (every lane all of them when R1 < W)
