@@ -92,6 +92,7 @@ struct qpe_state {
     int64_t max_tile_bytes = 0;
     int order = 1;   // Trotter order
     int term_order = 0;        // QPE_OPT_TERM_ORDER of the chain
+    int qift_form = 0;         // QPE_OPT_QIFT_FORM
     int plan_term_order = -1;  // term order of the cached plan
     uint64_t evolved_to = 0;   // blocks < evolved_to hold U^j psi (chained qpe_time_evolve_blocks)
     bool scale_pending = false;
@@ -571,6 +572,12 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
             s->term_order = (int)value;
             s->plan_version = ~0ull;
             break;
+        case QPE_OPT_QIFT_FORM:
+            require(value >= 0 && value <= 3, QPE_ERR_ARG, "QIFT form must be 0 (auto), 1, 2 or 3");
+            require(qpe::qift_form_available((int)value, (int)s->m), QPE_ERR_ARG,
+                    "this QIFT form does not cover 2^precision (register DFTs: 2..1024; matrix cores: 16..64)");
+            s->qift_form = (int)value;
+            break;
         case QPE_OPT_MAX_TILE_BYTES:
             require(value >= 0, QPE_ERR_ARG, "negative tile budget");
             s->max_tile_bytes = value;
@@ -850,7 +857,7 @@ int qpe_qift(qpe_state_t s) {
         qpe::hip_check(hipEventCreate(&e1), "event");
         qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
         qpe::qift(s->dtype, s->amp, (int)s->m, s->D, s->scale, s->pk_partial, &s->pk_grid, s->twiddles,
-                  (size_t)s->max_lds, s->stream);
+                  (size_t)s->max_lds, s->stream, kPkGridMax, s->qift_form);
         qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
         qpe::hip_check(hipEventSynchronize(e1), "qift sync");
         float ms = 0.f;
@@ -877,7 +884,7 @@ int qpe_qift_noalloc(qpe_state_t s, double *probs_device) {
         require(pe == hipSuccess && (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged), QPE_ERR_ARG,
                 "probs must point to device memory (2^precision doubles)");
         qpe::qift_noalloc(s->dtype, s->amp, (int)s->m, s->D, s->scale, s->twiddles, (size_t)s->max_lds, probs_device,
-                          s->stream);
+                          s->stream, s->qift_form);
         qpe::hip_check(hipStreamSynchronize(s->stream), "qift sync");
         s->scale_pending = false;   // the QIFT applied the factors and reset them to 1
         s->pk_valid = false;        // no partial sums of this QIFT are kept with the state

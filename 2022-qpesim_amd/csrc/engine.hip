@@ -1300,6 +1300,139 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
     }
 }
 
+// ---- QIFT as a matrix product on the matrix cores (2^m in 16..64) ------------------
+// Y = F X per 16-column block, F[k][j] = e^{-2 pi i jk/M} scale[j] / sqrt(M), in real
+// form: rows 2k, 2k+1 of Y~ are Re, Im of Y[k]; F~ has 2x2 blocks [[Fr, -Fi], [Fi, Fr]].
+// One wave owns a block: v_mfma_*_16x16x4 with A = a 16x4 slice of F~ (lane i + 16 kk
+// holds row i, column kk), B = 4 rows of X~ (lane n + 16 kk holds row kk, column n) and
+// the 16x16 result in 4 registers per lane n + 16 g: register r holds row 4 g + r (FP32)
+// or row g + 4 r (FP64) of column n.  8 M^2 flops per column against
+// 2 * sizeof(amplitude) * M bytes.
+template <typename R> struct MfmaAcc;
+template <> struct MfmaAcc<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ type fma(float a, float b, type c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MfmaAcc<double> {
+    typedef double type __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ type fma(double a, double b, type c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+// complex row of Y that lane group g stores as its h-th value (h = 0, 1) of 16-row block rb
+template <typename R> __device__ __forceinline__ int mfma_out_row(int rb, int h, int g) {
+    if constexpr (sizeof(R) == 4) return 8 * rb + 2 * g + h;                       // registers (2h, 2h+1)
+    else return 8 * rb + (g >> 1) + 4 * (g & 1) + 2 * h;   // Re and Im from lanes g and g ^ 1
+}
+
+template <int LOGM> struct MfmaShape {
+    static constexpr int M = 1 << LOGM;
+    static constexpr int RB = M / 8;   // 16-row blocks of Y~ (2M rows)
+    static constexpr int KS = M / 2;   // 4-row steps of X~ (2M rows)
+    static constexpr int T = 256;      // four waves, one 16-column block each at a time
+    template <typename R> static constexpr size_t lds_bytes() { return (size_t)RB * KS * 64 * sizeof(R); }
+};
+
+template <typename R, int LOGM>
+__global__ void __launch_bounds__(256)
+qift_mfma_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, const double2 *twiddles,
+                 double *pk_partial) {
+    using S = MfmaShape<LOGM>;
+    using V = typename Vec<R>::type;
+    using A4 = typename MfmaAcc<R>::type;
+    constexpr int M = S::M, RB = S::RB, KS = S::KS;
+    extern __shared__ __align__(16) unsigned char smem[];
+    R *Fl = reinterpret_cast<R *>(smem);   // [RB][KS][64]: each lane's A operand, in order
+    const double inv = 1.0 / sqrt((double)M);
+    for (int e = threadIdx.x; e < RB * KS * 64; e += S::T) {
+        const int ln = e & 63, rk = e >> 6, ks = rk % KS, rb = rk / KS;
+        const int row = 16 * rb + (ln & 15), col = 4 * ks + (ln >> 4);
+        const int k = row >> 1, a = row & 1, j = col >> 1, b = col & 1;
+        const int ex = (j * k) & (M - 1);
+        double2 z = twiddles[ex & (M / 2 - 1)];
+        if (ex >= M / 2) {
+            z.x = -z.x;
+            z.y = -z.y;
+        }
+        const double f = scale[j] * inv;
+        const double fr = z.x * f, fi = z.y * f;
+        Fl[e] = (R)(a == b ? fr : (a == 0 ? -fi : fi));
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    double acc[2 * RB], cmp[2 * RB];
+    static_for<0, 2 * RB>([&](auto i) { acc[i.value] = 0.0; cmp[i.value] = 0.0; });
+    const uint64_t nblk = (D + 15) / 16;
+    const uint64_t gw = (uint64_t)blockIdx.x * 4 + wave, nw = (uint64_t)gridDim.x * 4;
+    for (uint64_t blk = gw; blk < nblk; blk += nw) {
+        // opaque per block: keeps the row offsets from being hoisted into registers
+        uint64_t Dt = D;
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+s"(Dt), "+v"(lane)::"memory");
+        const int n = lane & 15, g = lane >> 4;
+        const uint64_t d = blk * 16 + n;
+        const bool live = d < Dt;
+        const V *src = state + (live ? d : 0) + (uint64_t)(g >> 1) * Dt;   // complex row 2 ks + g/2
+        V xb[KS];
+        static_for<0, KS>([&](auto ks) { xb[ks.value] = src[(uint64_t)(2 * ks.value) * Dt]; });
+        A4 C[RB];
+        static_for<0, RB>([&](auto rb) { C[rb.value] = A4{0, 0, 0, 0}; });
+        static_for<0, KS>([&](auto ks) {
+            const R bv = live ? ((g & 1) ? xb[ks.value].y : xb[ks.value].x) : (R)0;
+            static_for<0, RB>([&](auto rb) {
+                const R av = Fl[(rb.value * KS + ks.value) * 64 + lane];
+                C[rb.value] = MfmaAcc<R>::fma(av, bv, C[rb.value]);
+            });
+        });
+        V *dst = state + d;
+        static_for<0, RB>([&](auto rb) {
+            V y0, y1;
+            if constexpr (sizeof(R) == 4) {
+                y0.x = C[rb.value][0];
+                y0.y = C[rb.value][1];
+                y1.x = C[rb.value][2];
+                y1.y = C[rb.value][3];
+            } else {
+                // even g holds the real parts of its rows, odd g the imaginary parts:
+                // swap halves with the partner lane (g ^ 1, lane ^ 16)
+                const bool odd = (g & 1) != 0;
+                const R r0 = __shfl_xor(odd ? C[rb.value][0] : C[rb.value][2], 16);
+                const R r1 = __shfl_xor(odd ? C[rb.value][1] : C[rb.value][3], 16);
+                y0.x = odd ? r0 : C[rb.value][0];
+                y0.y = odd ? C[rb.value][2] : r0;
+                y1.x = odd ? r1 : C[rb.value][1];
+                y1.y = odd ? C[rb.value][3] : r1;
+            }
+            if (live) {
+                dst[(uint64_t)mfma_out_row<R>(rb.value, 0, g) * Dt] = y0;
+                dst[(uint64_t)mfma_out_row<R>(rb.value, 1, g) * Dt] = y1;
+            }
+            const double p0 = live ? (double)y0.x * (double)y0.x + (double)y0.y * (double)y0.y : 0.0;
+            const double p1 = live ? (double)y1.x * (double)y1.x + (double)y1.y * (double)y1.y : 0.0;
+            static_for<0, 2>([&](auto h) {
+                constexpr int i = 2 * rb.value + h.value;
+                const double x = h.value ? p1 : p0, a0 = acc[i], s2 = a0 + x;
+                if (fabs(a0) >= fabs(x)) cmp[i] += (a0 - s2) + x;
+                else cmp[i] += (x - s2) + a0;
+                acc[i] = s2;
+            });
+        });
+    }
+    // sum over the 16 columns (lanes n) of each row group; one row of partial sums per wave
+    const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4;
+    static_for<0, 2 * RB>([&](auto i) {
+        double v = acc[i.value] + cmp[i.value];
+        v += __shfl_xor(v, 1, 16);
+        v += __shfl_xor(v, 2, 16);
+        v += __shfl_xor(v, 4, 16);
+        v += __shfl_xor(v, 8, 16);
+        if (n == 0) pk_partial[gw * M + mfma_out_row<R>(i.value >> 1, i.value & 1, g)] = v;
+    });
+}
+
 __global__ void pk_reduce(const double *pk_partial, int grid, uint32_t M, double *probs) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= M) return;
@@ -1832,11 +1965,53 @@ static bool qift_reg(int m, void *state, uint64_t D, const double *scale, const 
     }
 }
 
+template <typename R, int LOGM>
+static bool launch_qift_mfma(void *state, uint64_t D, const double *scale, const double *twiddles, double *pk_partial,
+                             int max_grid, size_t max_lds, int *grid_out, hipStream_t s) {
+    using S = MfmaShape<LOGM>;
+    const size_t lds = S::template lds_bytes<R>();
+    if (lds > max_lds) return false;
+    const uint64_t nblk = (D + 15) / 16;
+    // one row of partial sums per wave (4 per workgroup)
+    int grid = (int)std::min<uint64_t>((nblk + 3) / 4, (uint64_t)std::min(1024, max_grid) / 4);
+    if (grid < 1) throw std::runtime_error("no workgroup fits the P(k) partial sums");
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&qift_mfma_kernel<R, LOGM>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((qift_mfma_kernel<R, LOGM>), dim3(grid), dim3(S::T), lds, s,
+                       reinterpret_cast<typename Vec<R>::type *>(state), D, scale,
+                       reinterpret_cast<const double2 *>(twiddles), pk_partial);
+    hip_check(hipGetLastError(), "qift launch");
+    *grid_out = grid * 4;
+    return true;
+}
+
+template <typename R>
+static bool qift_mfma(int m, void *state, uint64_t D, const double *scale, const double *twiddles, double *pk_partial,
+                      int max_grid, size_t max_lds, int *grid_out, hipStream_t s) {
+    switch (m) {
+    case 4: return launch_qift_mfma<R, 4>(state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s);
+    case 5: return launch_qift_mfma<R, 5>(state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s);
+    case 6: return launch_qift_mfma<R, 6>(state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s);
+    default: return false;
+    }
+}
+
+bool qift_form_available(int form, int m) {
+    if (form == QIFT_MFMA) return m >= 4 && m <= 6;
+    if (form == QIFT_REGISTER) return m >= 1 && m <= 10;
+    return true;
+}
+
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial, int *grid_out,
-          double *twiddles, size_t max_lds, hipStream_t s, int max_grid) {
+          double *twiddles, size_t max_lds, hipStream_t s, int max_grid, int form) {
+    if (form == QIFT_MFMA &&
+        (dtype == 0 ? qift_mfma<double>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
+                    : qift_mfma<float>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)))
+        return;
     // 2^m <= 1024: the register form; larger: radix-2 stages in LDS
-    if (dtype == 0 ? qift_reg<double>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
-                   : qift_reg<float>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s))
+    if (form != QIFT_LDS_RADIX2 &&
+        (dtype == 0 ? qift_reg<double>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
+                    : qift_reg<float>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)))
         return;
     const uint32_t M = 1u << m;
     const size_t tw_bytes = (size_t)std::max<uint32_t>(1, M / 2) * sizeof(double2);
@@ -1897,7 +2072,7 @@ double *scratch_address() {
 }  // namespace
 
 void qift_noalloc(int dtype, void *state, int m, uint64_t D, double *scale, double *twiddles, size_t max_lds,
-                  double *probs_dev, hipStream_t s) {
+                  double *probs_dev, hipStream_t s, int form) {
     const uint64_t M = 1ull << m;
     if (M > kNoAllocScratchDoubles)
         throw std::runtime_error("2^m above the allocation-free QIFT's scratch (" +
@@ -1905,7 +2080,7 @@ void qift_noalloc(int dtype, void *state, int m, uint64_t D, double *scale, doub
     double *part = scratch_address();
     int grid = 0;
     // every workgroup writes all 2^m of its partial sums, so the scratch needs no clearing
-    qift(dtype, state, m, D, scale, part, &grid, twiddles, max_lds, s, (int)(kNoAllocScratchDoubles / M));
+    qift(dtype, state, m, D, scale, part, &grid, twiddles, max_lds, s, (int)(kNoAllocScratchDoubles / M), form);
     reduce_pk(part, grid, m, probs_dev, s);
     // the QIFT applied the renormalisation factors: reset them to 1 (a fill kernel, no memset)
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)std::min<uint64_t>((M + 255) / 256, 1024)), dim3(256), 0, s, scale,

@@ -108,10 +108,16 @@ void evolve_in_place(const DevicePlan &dp, void *state, uint64_t block, uint64_t
 void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, double *scale,
                 double *partials, int nparts, hipStream_t s);
 
+// QIFT kernels: 0 picks per size; the others exist for 2^m in 1..1024 (register DFTs),
+// 16..64 (matrix cores) and any size (radix-2 stages in LDS)
+enum QiftForm { QIFT_AUTO = 0, QIFT_REGISTER = 1, QIFT_MFMA = 2, QIFT_LDS_RADIX2 = 3 };
+bool qift_form_available(int form, int m);
+
 // QIFT in place with the renormalisation factors applied, and P(k) partials.
-// max_grid caps the workgroups (one row of 2^m partial sums each in pk_partial).
+// max_grid caps the rows of 2^m partial sums written to pk_partial (*grid_out: rows written).
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial,
-          int *grid_out, double *twiddles, size_t max_lds, hipStream_t s, int max_grid = 1024);
+          int *grid_out, double *twiddles, size_t max_lds, hipStream_t s, int max_grid = 1024,
+          int form = QIFT_AUTO);
 
 // The same QIFT and P(k) with no allocation and no host transfer: the partial sums
 // in a static device array of kNoAllocScratchDoubles doubles (one per device), P(k)
@@ -120,7 +126,7 @@ void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double
 constexpr uint64_t kNoAllocScratchDoubles = 1ull << 20;   // 8 MiB
 constexpr int kMaxScratchDevices = 64;
 void qift_noalloc(int dtype, void *state, int m, uint64_t D, double *scale, double *twiddles, size_t max_lds,
-                  double *probs_dev, hipStream_t s);
+                  double *probs_dev, hipStream_t s, int form = QIFT_AUTO);
 void reduce_pk(const double *pk_partial, int grid, int m, double *probs_dev, hipStream_t s);
 
 // reference-order host data <-> device layout, with dtype conversion

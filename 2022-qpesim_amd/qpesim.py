@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libqpesim.so")
 COMPLEX128, COMPLEX64 = 0, 1
 LAYOUT_ALPHA_BETA, LAYOUT_SINGLE_STRING = 0, 1
 PP, PQ, PQQP, PQQR, PQRS = range(5)
-OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES, OPT_TROTTER_ORDER, OPT_TERM_ORDER = 0, 1, 2, 3, 4
+OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES, OPT_TROTTER_ORDER, OPT_TERM_ORDER, OPT_QIFT_FORM = 0, 1, 2, 3, 4, 5
+QIFT_AUTO, QIFT_REGISTER, QIFT_MFMA, QIFT_LDS_RADIX2 = 0, 1, 2, 3
 ERR_ARG, ERR_OOM, ERR_HIP, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
 
 # every entry point include/qpesim.h declares

@@ -269,10 +269,15 @@ enum qpe_option {
                                    2: symmetric second order, the first-order sequence at
                                    t/2 followed by the same sequence reversed at t/2 (no
                                    counterpart in the reference) */
-    QPE_OPT_TERM_ORDER = 4      /* term order of one step of time_evolve_over_terms:
+    QPE_OPT_TERM_ORDER = 4,     /* term order of one step of time_evolve_over_terms:
                                    0: applyUnitary_reordered (TEOT:374-562; default, the
                                    call TEOT:136 makes); 1: applyUnitary (TEOT:174-372, the
                                    call left commented out at TEOT:135) */
+    QPE_OPT_QIFT_FORM = 5       /* kernel of qpe_qift / qpe_qift_noalloc: 0 chosen by size
+                                   (default); 1 in-register DFTs (2^precision <= 1024);
+                                   2 DFT as a matrix product on the matrix cores
+                                   (2^precision 16..64); 3 radix-2 stages in LDS (any size).
+                                   All give the same QIFT within rounding. */
 };
 int qpe_state_set_option(qpe_state_t s, int option, int64_t value);
 
