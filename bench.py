@@ -262,6 +262,13 @@ def bench_batch(a):
 
 # ---- main -------------------------------------------------------------------------------------
 
+def qift_rate(ms, elem, M, D):
+    """HBM rate of the QIFT launch: every amplitude read and written once"""
+    gb = 2.0 * elem * M * D / 1e9
+    tbs = gb / ms if ms > 0 else 0.0
+    return {"ms": ms, "bytes": gb * 1e9, "TB/s": tbs, "frac": tbs * 1e3 / HBM_PEAK_GBS}
+
+
 def main(argv=None):
     a = parse(argv)
     if a.gpus != 1:
@@ -344,6 +351,7 @@ def main(argv=None):
         out.update({"qpe_wall_s_fp32": wall32, "evolve_ms_fp32": tm["evolve_ms"], "qift_ms_fp32": tm["qift_ms"],
                     "pk_ms_fp32": tm["pk_ms"], "updates_per_qpe": (M - 1) * upd_per_step,
                     "updates_per_s_qpe_fp32": (M - 1) * upd_per_step / (tm["evolve_ms"] * 1e-3)})
+        out["qift_fp32"] = qift_rate(tm["qift_ms"], 8, M, D)
         log(f"  complex64 QPE {wall32:.2f} s (evolve {tm['evolve_ms']:.0f} ms, qift {tm['qift_ms']:.1f} ms)")
         if a.save_pk:
             np.save(a.save_pk, P32)
@@ -366,6 +374,7 @@ def main(argv=None):
                               "evolve_ms_fp64": tm64["evolve_ms"], "sum_P": float(P64.sum()),
                               "argmax_P": int(P64.argmax())}))
             return
+        out["qift_fp64"] = qift_rate(tm64["qift_ms"], 16, M, D)
         out.update({"qpe_wall_s_fp64": wall64, "evolve_ms_fp64": tm64["evolve_ms"], "qift_ms_fp64": tm64["qift_ms"],
                     "value_fp64": (M - 1) * upd_per_step / (tm64["evolve_ms"] * 1e-3),
                     "sum_P_fp64": float(P64.sum()), "argmax_P": int(P64.argmax())})
