@@ -54,6 +54,9 @@ def parse(argv=None):
     ap.add_argument("--ref-pk", default=None, help="complex128 P(k) of the same instance from an earlier run (.npy)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2, help="Trotter steps timed on the CPU oracle")
+    ap.add_argument("--layout", choices=("alpha-beta", "single"), default="alpha-beta",
+                    help="register layout: the reference's alpha x beta blocks, or one interleaved string of "
+                         "2n spin-orbitals per determinant (C(2n,2e) amplitudes per block)")
     return ap.parse_args(argv)
 
 
@@ -290,6 +293,15 @@ def main(argv=None):
     psi0 = qpesim.initial_state(n, e, cfg["seed"], cfg["hf"])
     C = math.comb(n, e)
     D = C * C
+    layout = qpesim.LAYOUT_SINGLE_STRING if a.layout == "single" else qpesim.LAYOUT_ALPHA_BETA
+    if layout == qpesim.LAYOUT_SINGLE_STRING:
+        # the same state in the single-string register: the sector's amplitudes at their
+        # interleaved ranks with the reordering sign, every other entry zero
+        idx, sign = qpesim.single_string_map(n, e)
+        D = math.comb(2 * n, 2 * e)
+        psi_ss = np.zeros(D, np.complex128)
+        psi_ss[idx] = sign * psi0
+        psi0 = psi_ss
     order = cfg.get("order", 1)
     info32 = qpesim.plan_info(h, qpesim.COMPLEX64)
     # plan_info describes the first-order step; a second-order step applies every term twice (t/2 each)
@@ -298,9 +310,10 @@ def main(argv=None):
     passes_per_step = order * info32["passes"]
     log(f"workload {a.config}: n={2*n} spin-orbitals N={2*e} m={m} C={C} D={D} terms={h.term_counts()} "
         f"tau={tau:.6e}; plan c64: {info32}; runs: {plan}")
-    workload = (f"QPE {2*n} spin-orbitals, {2*e} electrons, m={m} ancillas ({M} x {C}^2 = {M*D} amplitudes, "
-                f"alpha x beta layout), {'second-order (symmetric)' if order == 2 else 'first-order'} Trotter, "
-                f"seed {cfg['seed']}")
+    shape = (f"{M} x {C}^2 = {M*D} amplitudes, alpha x beta layout" if layout == qpesim.LAYOUT_ALPHA_BETA else
+             f"{M} x C({2*n},{2*e}) = {M*D} amplitudes, single-string layout")
+    workload = (f"QPE {2*n} spin-orbitals, {2*e} electrons, m={m} ancillas ({shape}), "
+                f"{'second-order (symmetric)' if order == 2 else 'first-order'} Trotter, seed {cfg['seed']}")
     out = {"metric": METRIC, "value": None, "unit": "det-term updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "complex64 (fp32 state/rotations; fp64 angles, norms, P(k))",
@@ -312,7 +325,7 @@ def main(argv=None):
 
     sv32 = None
     if not a.fp64_only:
-        sv32 = qpesim.StateVector(m, n, e, qpesim.COMPLEX64)
+        sv32 = qpesim.StateVector(m, n, e, qpesim.COMPLEX64, layout)
         sv32.set_option(qpesim.OPT_ENGINE, a.engine)
         sv32.set_option(qpesim.OPT_TROTTER_ORDER, order)
         sv32.set(0, psi0 / math.sqrt(M))
@@ -330,7 +343,7 @@ def main(argv=None):
         bytes_launch = bytes_per_step / passes_per_step
         avg_launch_s = dev_ms * 1e-3 / (a.steps * passes_per_step)
         achieved = bytes_launch / avg_launch_s / 1e9
-        traffic = measured_traffic(f"{a.config}/c64")
+        traffic = measured_traffic(f"{a.config}/c64") if layout == qpesim.LAYOUT_ALPHA_BETA else None
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
                            "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
@@ -338,6 +351,9 @@ def main(argv=None):
                            "bytes_per_launch": bytes_launch, "launches_per_step": passes_per_step,
                            "avg_launch_us": avg_launch_s * 1e6,
                            "timing": "HIP events on the state's stream around the timed steps"}
+        if layout == qpesim.LAYOUT_SINGLE_STRING:
+            out["roofline"]["note"] = ("device time per step includes the two layout conversions "
+                                       "(single-string block j-1 -> alpha x beta, block j back)")
         if traffic:
             out["roofline"]["traffic_source"] = traffic.get("source")
             if "dram_bytes_per_launch" in traffic:
@@ -360,7 +376,7 @@ def main(argv=None):
 
     if plan["full_qpe_fp64"]:
         log("  full QPE, complex128")
-        sv64 = qpesim.StateVector(m, n, e, qpesim.COMPLEX128)
+        sv64 = qpesim.StateVector(m, n, e, qpesim.COMPLEX128, layout)
         sv64.set_option(qpesim.OPT_ENGINE, a.engine)
         sv64.set_option(qpesim.OPT_TROTTER_ORDER, order)
         wall64, tm64, P64 = full_qpe(sv64, h, psi0, M)

@@ -20,7 +20,7 @@ def bench():
 
 
 class FakeSV:
-    def __init__(self, m, n, e, dtype):
+    def __init__(self, m, n, e, dtype, layout=0):
         self.m, self.M, self.dtype = m, 1 << m, dtype
         self.ms = 0.0
 
