@@ -1866,41 +1866,58 @@ void BatchRun::run(hipStream_t s, bool use_graph) {
         return;
     }
     instantiate();
-    hip_check(hipGraphLaunch(graph, s), "graph launch");
+    for (uint64_t r = 0; r < graph_reps; ++r) hip_check(hipGraphLaunch(graph, s), "graph launch");
+    if (tail) hip_check(hipGraphLaunch(tail, s), "graph launch");
+}
+
+// steps kernels of the chain captured into one executable graph
+template <typename R>
+static hipGraphExec_t capture_steps(const BatchRun &b, uint64_t steps) {
+    hipStream_t cs;
+    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+    hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (uint64_t j = 0; j < steps; ++j) batch_step<R>(b, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e1 = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    hip_check(e1, "end capture");
+    hipGraphExec_t exec = nullptr;
+    hipError_t e2 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(e2, "graph instantiate");
+    return exec;
 }
 
 void BatchRun::instantiate() {
-    if (graph && graph_steps == (int)(M - 1)) return;
-    if (graph) (void)hipGraphExecDestroy(graph);
-    graph = nullptr;
+    if (graph || tail) return;
     if (dtype == 0)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<double>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     else
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<float>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    // capture every step of the chain; each step's kernels read the step from *jstep
-    hipStream_t cs;
-    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
-    hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
-    for (uint64_t j = 1; j < M; ++j) {
-        if (dtype == 0) batch_step<double>(*this, cs);
-        else batch_step<float>(*this, cs);
-    }
-    hipGraph_t g = nullptr;
-    hipError_t e1 = hipStreamEndCapture(cs, &g);
-    (void)hipStreamDestroy(cs);
-    hip_check(e1, "end capture");
-    hipError_t e2 = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    hip_check(e2, "graph instantiate");
-    graph_steps = (int)(M - 1);
+    // each step's kernels read the step from *jstep, so one graph of S steps replayed
+    // (M-1)/S times and a graph of the remaining steps run the whole chain
+    uint64_t per_step = 1;   // the finalize
+    for (const auto &L : launches)
+        if (L.blocks) ++per_step;
+    const uint64_t total = M - 1;
+    uint64_t S = std::max<uint64_t>(1, kMaxGraphLaunches / per_step);
+    if (S > total) S = total;
+    graph_steps = S;
+    graph_reps = S ? total / S : 0;
+    tail_steps = S ? total - graph_reps * S : 0;
+    if (graph_reps)
+        graph = dtype == 0 ? capture_steps<double>(*this, S) : capture_steps<float>(*this, S);
+    if (tail_steps)
+        tail = dtype == 0 ? capture_steps<double>(*this, tail_steps) : capture_steps<float>(*this, tail_steps);
 }
 
 void BatchRun::release() {
     if (graph) (void)hipGraphExecDestroy(graph);
-    graph = nullptr;
-    graph_steps = 0;
+    if (tail) (void)hipGraphExecDestroy(tail);
+    graph = tail = nullptr;
+    graph_steps = graph_reps = tail_steps = 0;
     void *ptrs[] = {args, work, fin, jstep, done};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);

@@ -79,13 +79,17 @@ struct BatchRun {
         size_t lds;                // dynamic LDS: the largest of its members' pass k
     };
     std::vector<Launch> launches;  // one step
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;           // steps captured in `graph`
+    // The chain of M-1 steps as graph launches: `graph` holds graph_steps steps and runs
+    // graph_reps times, `tail` the remaining tail_steps; a graph holds at most
+    // kMaxGraphLaunches kernel launches
+    static constexpr uint64_t kMaxGraphLaunches = 1u << 15;
+    hipGraphExec_t graph = nullptr, tail = nullptr;
+    uint64_t graph_steps = 0, graph_reps = 0, tail_steps = 0;
 
     void build(const std::vector<BatchMember> &members, uint64_t M, hipStream_t s);
-    // captures steps 1..M-1 into one graph (once)
+    // captures the chain's graphs (once)
     void instantiate();
-    // runs steps 1..M-1; with use_graph the whole chain is one graph launch
+    // runs steps 1..M-1; with use_graph as graph launches
     void run(hipStream_t s, bool use_graph);
     void release();
     ~BatchRun() { release(); }
