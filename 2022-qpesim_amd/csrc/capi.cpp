@@ -791,61 +791,170 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
 
 // ---- autotuner ------------------------------------------------------------------
 
+namespace {
+// one timed configuration of an instance
+struct TuneCandidate {
+    int64_t budget;   // QPE_OPT_MAX_TILE_BYTES
+    int engine;       // QPE_OPT_ENGINE
+    double us;        // one Trotter step
+    int wgs;          // workgroups of the widest pass
+    int passes;       // passes per step
+};
+
+// Times one Trotter step of every candidate tile width (lines per workgroup, from the widest
+// that fits down to 1) and grouping of instance (s, h) on a two-block scratch state.
+std::vector<TuneCandidate> tune_candidates(qpe_state_t s, qpe_hamiltonian_t h) {
+    check_state(s);
+    check_ham(h);
+    require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
+    require(s->layout == QPE_LAYOUT_ALPHA_BETA, QPE_ERR_ARG, "the autotuner needs the alpha x beta layout");
+    const size_t full = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
+    const size_t line = (size_t)s->C * s->elem;
+    std::vector<size_t> budgets;
+    for (size_t w = std::min<size_t>(full / line, 64); w >= 1; w /= 2) budgets.push_back(std::min(full, w * line + 64));
+    qpe_state_t t = nullptr;
+    int rc = qpe_state_create(&t, 1, s->n, s->e, s->dtype);
+    if (rc != QPE_OK) throw ApiError(rc, "autotune scratch state: " + std::string(qpe_last_error()));
+    std::unique_ptr<qpe_state, int (*)(qpe_state_t)> guard(t, qpe_state_destroy);
+    const double one[2] = {1.0, 0.0};
+    std::vector<TuneCandidate> out;
+    for (size_t b : budgets) {
+        for (int engine = 0; engine < 2; ++engine) {
+            t->max_tile_bytes = (int64_t)b;
+            t->engine = engine;
+            t->order = s->order;
+            t->term_order = s->term_order;
+            t->plan_version = ~0ull;
+            t->plan.reset();
+            double us = 0.0;
+            bool ok = true;
+            for (int rep = 0; rep < 3 && ok; ++rep) {   // first run builds the plan
+                if (qpe_state_set(t, 0, 1, one) != QPE_OK || qpe_time_evolve_over_terms(t, h) != QPE_OK) ok = false;
+                else if (rep > 0) us += t->t_evolve * 1e3 / 2.0;
+            }
+            if (!ok) continue;   // e.g. a budget too small for a mixed double excitation
+            TuneCandidate c;
+            c.budget = (int64_t)b;
+            c.engine = engine;
+            c.us = us;
+            c.wgs = 0;
+            for (const auto &ps : t->plan->plan.passes) c.wgs = std::max(c.wgs, ps.item_count);
+            c.passes = (int)t->plan->plan.passes.size();
+            out.push_back(c);
+        }
+    }
+    require(!out.empty(), QPE_ERR_ARG, "no tile configuration fits this instance");
+    return out;
+}
+
+void apply_choice(qpe_state_t s, const TuneCandidate &c, double *result) {
+    s->max_tile_bytes = c.budget;
+    s->engine = c.engine;
+    s->plan_version = ~0ull;
+    if (result) {
+        result[0] = (double)c.budget;
+        result[1] = (double)c.engine;
+        result[2] = c.us;
+    }
+}
+}  // namespace
+
 int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *result) {
     return guarded([&] {
-        check_state(s);
-        check_ham(h);
-        require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
         require(share_cus >= 1, QPE_ERR_ARG, "share_cus must be positive");
-        const size_t full = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
-        const size_t line = (size_t)s->C * s->elem;
-        // candidate tile widths (lines per workgroup) from the widest that fits down to 1
-        std::vector<size_t> budgets;
-        for (size_t w = std::min<size_t>(full / line, 64); w >= 1; w /= 2) budgets.push_back(std::min(full, w * line + 64));
-        // a two-block scratch state of the same shape: one Trotter step per trial
-        qpe_state_t t = nullptr;
-        int rc = qpe_state_create(&t, 1, s->n, s->e, s->dtype);
-        if (rc != QPE_OK) throw ApiError(rc, "autotune scratch state: " + std::string(qpe_last_error()));
-        std::unique_ptr<qpe_state, int (*)(qpe_state_t)> guard(t, qpe_state_destroy);
-        const double one[2] = {1.0, 0.0};
-        double best = 1e300, best_us = 0.0;
-        int64_t best_budget = 0;
-        int best_engine = 0;
-        for (size_t b : budgets) {
-            for (int engine = 0; engine < 2; ++engine) {
-                t->max_tile_bytes = (int64_t)b;
-                t->engine = engine;
-                t->order = s->order;
-                t->term_order = s->term_order;
-                t->plan_version = ~0ull;
-                t->plan.reset();
-                double us = 0.0;
-                bool ok = true;
-                for (int rep = 0; rep < 3 && ok; ++rep) {   // first run builds the plan
-                    if (qpe_state_set(t, 0, 1, one) != QPE_OK || qpe_time_evolve_over_terms(t, h) != QPE_OK) ok = false;
-                    else if (rep > 0) us += t->t_evolve * 1e3 / 2.0;
-                }
-                if (!ok) continue;   // e.g. a budget too small for a mixed double excitation
-                int wgs = 0;
-                for (const auto &ps : t->plan->plan.passes) wgs = std::max(wgs, ps.item_count);
-                const double cost = us * std::max(1.0, (double)wgs / (double)share_cus);
-                if (cost < best) {
-                    best = cost;
-                    best_us = us;
-                    best_budget = (int64_t)b;
-                    best_engine = engine;
-                }
+        const std::vector<TuneCandidate> cands = tune_candidates(s, h);
+        size_t best = 0;
+        double best_cost = 1e300;
+        for (size_t i = 0; i < cands.size(); ++i) {
+            const double cost = cands[i].us * std::max(1.0, (double)cands[i].wgs / (double)share_cus);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best = i;
             }
         }
-        require(best < 1e300, QPE_ERR_ARG, "no tile configuration fits this instance");
-        s->max_tile_bytes = best_budget;
-        s->engine = best_engine;
-        s->plan_version = ~0ull;
-        if (result) {
-            result[0] = (double)best_budget;
-            result[1] = (double)best_engine;
-            result[2] = best_us;
+        apply_choice(s, cands[best], result);
+    });
+}
+
+int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, double *results) {
+    return guarded([&] {
+        require(states != nullptr && hams != nullptr && count >= 1, QPE_ERR_ARG, "empty batch");
+        std::vector<std::vector<TuneCandidate>> cands;
+        for (int i = 0; i < count; ++i) cands.push_back(tune_candidates(states[i], hams[i]));
+        // fastest candidate of instance i with at most max_passes passes per step (none: the fewest passes)
+        auto fastest = [&](int i, int max_passes) {
+            size_t best = 0;
+            bool found = false;
+            for (size_t k = 0; k < cands[i].size(); ++k) {
+                const TuneCandidate &c = cands[i][k];
+                if (c.passes > max_passes) continue;
+                if (!found || c.us < cands[i][best].us) best = k;
+                found = true;
+            }
+            if (!found)
+                for (size_t k = 0; k < cands[i].size(); ++k)
+                    if (cands[i][k].passes < cands[i][best].passes) best = k;
+            return best;
+        };
+        // Two policies, each timed as one batched step on two-block scratch states:
+        //  A  every instance at its fewest passes (the fastest of those): the fewest launches;
+        //  B  the instance whose fastest step is slowest at that fastest candidate, every other
+        //     instance at its fastest candidate with no more passes than that one.
+        std::vector<size_t> pol_a(count), pol_b(count);
+        for (int i = 0; i < count; ++i) {
+            int minp = INT32_MAX;
+            for (const auto &c : cands[i]) minp = std::min(minp, c.passes);
+            pol_a[i] = fastest(i, minp);
         }
+        int crit = 0;
+        double crit_us = -1.0;
+        for (int i = 0; i < count; ++i) {
+            const double us = cands[i][fastest(i, INT32_MAX)].us;
+            if (us > crit_us) {
+                crit_us = us;
+                crit = i;
+            }
+        }
+        const size_t cbest = fastest(crit, INT32_MAX);
+        for (int i = 0; i < count; ++i) pol_b[i] = i == crit ? cbest : fastest(i, cands[crit][cbest].passes);
+        auto time_policy = [&](const std::vector<size_t> &pol) {
+            std::vector<qpe_state_t> t(count, nullptr);
+            auto cleanup = [&] {
+                for (qpe_state_t x : t)
+                    if (x) qpe_state_destroy(x);
+            };
+            try {
+                const double one[2] = {1.0, 0.0};
+                for (int i = 0; i < count; ++i) {
+                    int rc = qpe_state_create(&t[i], 1, states[i]->n, states[i]->e, states[i]->dtype);
+                    if (rc != QPE_OK) throw ApiError(rc, "autotune scratch state: " + std::string(qpe_last_error()));
+                    t[i]->max_tile_bytes = cands[i][pol[i]].budget;
+                    t[i]->engine = cands[i][pol[i]].engine;
+                    t[i]->order = states[i]->order;
+                    t[i]->term_order = states[i]->term_order;
+                    t[i]->renorm = states[i]->renorm;
+                }
+                double best_ms = 1e300;
+                for (int rep = 0; rep < 3; ++rep) {   // the first run builds plans and the graph
+                    for (int i = 0; i < count; ++i)
+                        if (qpe_state_set(t[i], 0, 1, one) != QPE_OK)
+                            throw ApiError(QPE_ERR_STATE, "autotune scratch state: " + std::string(qpe_last_error()));
+                    double ms = 0.0;
+                    int rc = qpe_batch_time_evolve(t.data(), hams, count, 0, &ms);
+                    if (rc != QPE_OK) throw ApiError(rc, "autotune batch: " + std::string(qpe_last_error()));
+                    if (rep > 0) best_ms = std::min(best_ms, ms);
+                }
+                cleanup();
+                return best_ms;
+            } catch (...) {
+                cleanup();
+                throw;
+            }
+        };
+        const double ms_a = time_policy(pol_a);
+        const double ms_b = pol_b == pol_a ? ms_a : time_policy(pol_b);
+        const std::vector<size_t> &pol = ms_b < ms_a ? pol_b : pol_a;
+        for (int i = 0; i < count; ++i) apply_choice(states[i], cands[i][pol[i]], results ? results + 3 * i : nullptr);
     });
 }
 

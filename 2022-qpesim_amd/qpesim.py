@@ -39,7 +39,7 @@ EXPORTS = [
     "qpe_hamiltonian_ingest_broombridge", "qpe_hamiltonian_ingest_broombridge_text", "qpe_hamiltonian_initial_state",
     "qpe_hamiltonian_identity_term", "qpe_hamiltonian_get_sizes", "qpe_state_ingest_broombridge", "qpe_energy_estimate",
     "qpe_time_evolve_iqpe", "qpe_state_apply_gate", "qpe_state_apply_controlled_gate", "qpe_qift_noalloc",
-    "qpe_state_create_layout",
+    "qpe_state_create_layout", "qpe_autotune_batch",
 ]
 BATCH_EAGER = 1
 
@@ -94,6 +94,7 @@ def load():
         "qpe_state_apply_controlled_gate": [vp, P(dbl), ctypes.c_uint, ctypes.c_uint],
         "qpe_qift_noalloc": [vp, vp],
         "qpe_state_create_layout": [P(vp), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ci, ci],
+        "qpe_autotune_batch": [P(vp), P(vp), ci, P(dbl)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -370,6 +371,22 @@ def autotune(sv, h, share_cus=256):
     out = np.zeros(3, np.float64)
     _call("qpe_autotune", sv.handle, h.handle, int(share_cus), _dptr(out))
     return {"tile_bytes": int(out[0]), "engine": int(out[1]), "us_per_step": float(out[2])}
+
+
+def autotune_batch(states, hams):
+    """Batching form's autotuner: times every candidate of every instance, then gives
+    the instance with the slowest fastest step its fastest candidate and every other
+    instance its fastest candidate with no more passes per step.  Returns one
+    {tile_bytes, engine, us_per_step} per instance."""
+    n = len(states)
+    if n != len(hams) or n == 0:
+        raise ValueError("states and hams must be non-empty and of equal length")
+    sa = (ctypes.c_void_p * n)(*[sv.handle.value for sv in states])
+    ha = (ctypes.c_void_p * n)(*[h.handle.value for h in hams])
+    out = np.zeros(3 * n, np.float64)
+    _call("qpe_autotune_batch", sa, ha, n, _dptr(out))
+    return [{"tile_bytes": int(out[3 * i]), "engine": int(out[3 * i + 1]), "us_per_step": float(out[3 * i + 2])}
+            for i in range(n)]
 
 
 def energy_estimate(probs, precision, identity, timestep):

@@ -189,7 +189,7 @@ def bench_batch(a):
     specs = qpesim.batch_instances()
     m = a.m if a.m is not None else specs[0]["m"]
     M = 1 << m
-    hams, psis, updates, work = [], [], [], []
+    hams, psis, updates = [], [], []
     for sp in specs:
         one, two = qpesim.synthetic_integrals(sp["n"], sp["seed"])
         h, _ = qpesim.build_hamiltonian(sp["n"], sp["e"], one, two)
@@ -197,18 +197,13 @@ def bench_batch(a):
         hams.append(h)
         psis.append(qpesim.initial_state(sp["n"], sp["e"], sp["seed"], sp["hf"]))
         updates.append(info["pair_updates"] + info["diag_updates"])
-        work.append(info["pass_bytes"])
     tot_upd = (M - 1) * sum(updates)
     log(f"batch: {len(specs)} instances, m={m}, C from {min(math.comb(s['n'], s['e']) for s in specs)} "
         f"to {max(math.comb(s['n'], s['e']) for s in specs)}, {tot_upd:.3e} updates per QPE batch")
 
     def make(dtype, tune):
         svs = [qpesim.StateVector(m, sp["n"], sp["e"], dtype) for sp in specs]
-        choices = []
-        if tune:
-            wsum = float(sum(work))
-            for sv, h, w in zip(svs, hams, work):
-                choices.append(qpesim.autotune(sv, h, share_cus=max(1, int(round(256 * w / wsum)))))
+        choices = qpesim.autotune_batch(svs, hams) if tune else []
         return svs, choices
 
     def run(svs, eager=False):
@@ -248,7 +243,7 @@ def bench_batch(a):
         "batch_evolve_ms": ev_ms, "qift_pk_s": post_s, "updates_per_batch": tot_upd,
     }
     if not a.no_fp64:
-        svs64, _ = make(qpesim.COMPLEX128, False)
+        svs64, _ = make(qpesim.COMPLEX128, True)
         run(svs64)
         ms64 = run(svs64)
         dps = []

@@ -227,6 +227,16 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
  * result (may be NULL) receives {tile bytes, engine, microseconds per step}. */
 int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *result);
 
+/* Autotuner of the batching form (no counterpart in the reference): times every candidate
+ * of every instance as qpe_autotune does, then times one batched step of two policies on
+ * two-block scratch states and sets the faster: (A) every instance at its fewest passes per
+ * step (a launch of the batch runs pass k of every member, so this gives the fewest
+ * launches); (B) the instance whose fastest step is slowest at that candidate and every
+ * other instance at its fastest candidate with no more passes than that one.
+ * results (may be NULL): [count][3] as qpe_autotune's result.  Call before
+ * qpe_batch_time_evolve. */
+int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, double *results);
+
 /* StateVector::qift(), cpp/state_vector.hpp:115-179: in place, forward DFT
  * (e^{-2 pi i jk/M}) along the ancilla axis, natural order, divided by sqrt(M). */
 int qpe_qift(qpe_state_t s);

@@ -116,3 +116,27 @@ def test_batch_rejects_mixed_types_and_duplicates():
     assert ei.value.code == qpesim.ERR_ARG
     a[0].close()
     b[0].close()
+
+
+def test_batch_autotuner_choices():
+    """qpe_autotune_batch sets a valid configuration on every instance, and the tuned
+    batch still equals the tuned instances run one by one."""
+    m = 3
+    say("batch autotuner")
+    inst = _instances(m, qpesim.COMPLEX64)
+    svs, hams = [i[0] for i in inst], [i[1] for i in inst]
+    res = qpesim.autotune_batch(svs, hams)
+    assert len(res) == len(inst)
+    passes = []
+    for (sv, h, _), r in zip(inst, res):
+        assert r["tile_bytes"] > 0 and r["engine"] in (0, 1) and r["us_per_step"] > 0.0
+        qpesim.time_evolve_over_terms(sv, h)
+        passes.append(sv.timings()["passes_per_step"])
+    say(f"  choices {res}, passes per step {passes}")
+    single = _all(inst)
+    _reset(inst, m)
+    qpesim.batch_time_evolve(svs, hams)
+    for a, b in zip(single, _all(inst)):
+        assert np.array_equal(a, b)
+    for sv in svs:
+        sv.close()
