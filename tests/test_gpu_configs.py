@@ -46,6 +46,38 @@ def test_n32_fp32_form_against_fp64_gpu_63_steps():
     assert abs(P64.sum() - 1.0) < 1e-12
 
 
+# ---------------------------------------------------------------- n=28, N=10 (C=2002), FP64, m=8
+def test_n28_fp64_full_qpe_qift_and_norms():
+    """The plain FP64 workload at its full size (m=8, 255 Trotter steps; per-step parity
+    with the oracle is tested at this block size in test_gpu_parity): every block keeps
+    block 0's norm, the QIFT of the evolved register equals numpy's FFT of the same
+    amplitudes (64 determinants of every block) within 1e-12, and P(k) sums
+    to 1."""
+    n, e, m, seed = 14, 5, 8, 1
+    M = 1 << m
+    one, two = qpesim.synthetic_integrals(n, seed)
+    h, _ = qpesim.build_hamiltonian(n, e, one, two)
+    psi = qpesim.initial_state(n, e, seed, 0.9)
+    say("n=28 N=10 complex128 m=8: full chain, QIFT against numpy on a slice")
+    sv = qpesim.StateVector(m, n, e, qpesim.COMPLEX128)
+    D = sv.D
+    sv.set(0, psi / math.sqrt(M))
+    qpesim.time_evolve_over_terms(sv, h)
+    norms = sv.block_norms()
+    assert np.max(np.abs(norms - norms[0])) < 1e-12
+    c0 = D // 3
+    X = np.stack([sv.get(j * D + c0, 64) for j in range(M)])
+    sv.qift()
+    Y = np.stack([sv.get(j * D + c0, 64) for j in range(M)])
+    P = sv.identify_phase_amplitudes()
+    sv.close()
+    Yr = np.fft.fft(X, axis=0) / math.sqrt(M)
+    l2 = rel_l2(Y, Yr)
+    say(f"  norms spread {np.max(np.abs(norms - norms[0])):.1e}  QIFT slice L2 {l2:.2e}  sum P - 1 {P.sum() - 1:.1e}")
+    assert l2 < 1e-12
+    assert abs(P.sum() - 1.0) < 1e-12
+
+
 # ---------------------------------------------------------------- 48-instance batching form
 def _batch(specs, m, dtype):
     inst = []
