@@ -1141,11 +1141,20 @@ template <int N, int LEN, typename C2>
 __device__ __forceinline__ void dft_stage(C2 (&a)[N], const C2 *tw, int M) {
     constexpr int H = LEN / 2;
     static_for<0, H>([&](auto kk) {
+        // kk = 0: the twiddle is 1; kk = LEN/4: it is -i (exact, no multiply)
+        constexpr bool unit = kk.value == 0, minus_i = 4 * kk.value == LEN;
         const C2 z = tw[kk.value * (M / LEN)];
         static_for<0, N / LEN>([&](auto g) {
             constexpr int i0 = g.value * LEN + kk.value, i1 = i0 + H;
             const C2 u = a[i0];
-            const C2 t = kk.value ? cmul(a[i1], z) : a[i1];
+            C2 t = a[i1];
+            if constexpr (minus_i) {
+                const auto x = t.x;
+                t.x = t.y;
+                t.y = -x;
+            } else if constexpr (!unit) {
+                t = cmul(t, z);
+            }
             a[i0] = cadd(u, t);
             a[i1] = csub(u, t);
         });
