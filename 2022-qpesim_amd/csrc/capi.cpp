@@ -93,6 +93,7 @@ struct qpe_state {
     int order = 1;   // Trotter order
     int term_order = 0;        // QPE_OPT_TERM_ORDER of the chain
     int qift_form = 0;         // QPE_OPT_QIFT_FORM
+    int pk_rows = 0;           // rows of pk_partial
     int plan_term_order = -1;  // term order of the cached plan
     uint64_t evolved_to = 0;   // blocks < evolved_to hold U^j psi (chained qpe_time_evolve_blocks)
     bool scale_pending = false;
@@ -368,7 +369,9 @@ int qpe_state_create_layout(qpe_state_t *out, unsigned precision, unsigned num_o
         qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->norms), s->M * sizeof(double)), "hipMalloc(norms)");
         qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->scale), s->M * sizeof(double)), "hipMalloc(scale)");
         qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->probs), s->M * sizeof(double)), "hipMalloc(probs)");
-        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->pk_partial), (size_t)kPkGridMax * s->M * sizeof(double)),
+        // rows of 2^m P(k) partial sums: up to kPkGridMax, at most 64 MB in all
+        s->pk_rows = (int)std::min<uint64_t>(kPkGridMax, std::max<uint64_t>(8, (8ull << 20) / s->M));
+        qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->pk_partial), (size_t)s->pk_rows * s->M * sizeof(double)),
                        "hipMalloc(pk)");
         qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->norm_partials), kNormParts * sizeof(double)), "hipMalloc");
         qpe::hip_check(hipMalloc(reinterpret_cast<void **>(&s->status), sizeof(int)), "hipMalloc(status)");
@@ -961,12 +964,31 @@ int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams,
 int qpe_qift(qpe_state_t s) {
     return guarded([&] {
         check_state(s);
+        // 2^m > 1024: the QIFT's second step writes a second register (freed again below);
+        // without room for it, 2^m <= 4096 falls back to the radix-2 stages in LDS
+        int form = s->qift_form;
+        void *work = nullptr;
+        if (qpe::qift_needs_work(form, (int)s->m)) {
+            const size_t bytes = s->M * s->D * s->elem;
+            size_t free_b = 0, total_b = 0;
+            qpe::hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            if (bytes + (64ull << 20) < free_b && hipMalloc(&work, bytes) == hipSuccess) {
+                // room for the second register
+            } else {
+                (void)hipGetLastError();
+                work = nullptr;
+                require(form == qpe::QIFT_AUTO && s->m <= 12, QPE_ERR_OOM,
+                        "the QIFT of 2^precision > 1024 needs a second register's worth of free device memory");
+                form = qpe::QIFT_LDS_RADIX2;
+            }
+        }
+        std::unique_ptr<void, hipError_t (*)(void *)> work_guard(work, hipFree);
         hipEvent_t e0, e1;
         qpe::hip_check(hipEventCreate(&e0), "event");
         qpe::hip_check(hipEventCreate(&e1), "event");
         qpe::hip_check(hipEventRecord(e0, s->stream), "event record");
         qpe::qift(s->dtype, s->amp, (int)s->m, s->D, s->scale, s->pk_partial, &s->pk_grid, s->twiddles,
-                  (size_t)s->max_lds, s->stream, kPkGridMax, s->qift_form);
+                  (size_t)s->max_lds, s->stream, s->pk_rows, form, work);
         qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
         qpe::hip_check(hipEventSynchronize(e1), "qift sync");
         float ms = 0.f;
@@ -986,6 +1008,13 @@ int qpe_qift_noalloc(qpe_state_t s, double *probs_device) {
         require(probs_device != nullptr, QPE_ERR_ARG, "null probs");
         require(s->M <= qpe::kNoAllocScratchDoubles, QPE_ERR_ARG,
                 "2^precision exceeds the allocation-free QIFT's limit of 2^20 outcomes");
+        int form = s->qift_form;
+        if (qpe::qift_needs_work(form, (int)s->m)) {
+            // the register form above 1024 outcomes needs a second register: radix-2 stages instead
+            require(form == qpe::QIFT_AUTO && s->m <= 12, QPE_ERR_ARG,
+                    "the allocation-free QIFT covers 2^precision <= 4096");
+            form = qpe::QIFT_LDS_RADIX2;
+        }
         hipPointerAttribute_t at;
         std::memset(&at, 0, sizeof(at));
         const hipError_t pe = hipPointerGetAttributes(&at, probs_device);
@@ -993,7 +1022,7 @@ int qpe_qift_noalloc(qpe_state_t s, double *probs_device) {
         require(pe == hipSuccess && (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged), QPE_ERR_ARG,
                 "probs must point to device memory (2^precision doubles)");
         qpe::qift_noalloc(s->dtype, s->amp, (int)s->m, s->D, s->scale, s->twiddles, (size_t)s->max_lds, probs_device,
-                          s->stream, s->qift_form);
+                          s->stream, form);
         qpe::hip_check(hipStreamSynchronize(s->stream), "qift sync");
         s->scale_pending = false;   // the QIFT applied the factors and reset them to 1
         s->pk_valid = false;        // no partial sums of this QIFT are kept with the state

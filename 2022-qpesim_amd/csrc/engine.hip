@@ -1177,25 +1177,51 @@ __device__ __forceinline__ void reg_dft(C2 (&v)[1 << L], const C2 *tw, int M) {
     }
 }
 
-template <typename R, int LOGM>
+// Where one launch of the register-form kernel reads and writes.  Group g (blockIdx.y)
+// takes the 2^LOGM-point DFT of the rows g*gi + j*ji of src (j < 2^LOGM) and writes its
+// output k to row g*go + k*ko of dst; rows are D amplitudes apart.  The inputs are scaled
+// by scale[row] (use_scale), output k of group g is multiplied by e^{-2 pi i g k / Mt}
+// (group_twiddle) and by f, and |output|^2 is summed into P(g*gk + k*kk) (emit_pk).  The
+// sub-DFT's twiddles e^{-2 pi i k / 2^LOGM} are tw_full[k * tw_step], tw_full holding
+// e^{-2 pi i k / Mt} for k < Mt / 2.  One launch with one group is the whole QIFT for
+// 2^m <= 1024; two launches with 2^m = R1 * R2 groups split a larger one (four-step).
+struct QiftMap {
+    uint64_t gi, ji, go, ko, gk, kk;
+    uint32_t Mt, tw_step;
+    int group_twiddle, use_scale, emit_pk, pad;
+    double f;
+};
+
+// SPLIT: one of the two launches of a 2^m > 1024 QIFT (groups, strides, group twiddles);
+// otherwise the whole QIFT of one launch (the map's defaults, fixed at compile time)
+template <typename R, int LOGM, bool SPLIT>
 __global__ void __launch_bounds__((QiftShape<R, LOGM>::T))
-qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, const double2 *twiddles,
-                double *pk_partial) {
+qift_reg_kernel(const typename Vec<R>::type *src, typename Vec<R>::type *dst, uint64_t D, QiftMap mp,
+                const double *scale, const double2 *tw_full, double *pk_partial) {
     using Q = QiftShape<R, LOGM>;
     using V = typename Vec<R>::type;
     constexpr int M = Q::M, R1 = Q::R1, R2 = Q::R2, W = Q::W;
+    if constexpr (!SPLIT) {
+        mp.gi = 0; mp.ji = 1; mp.go = 0; mp.ko = 1; mp.gk = 0; mp.kk = 1;
+        mp.tw_step = 1;
+        mp.group_twiddle = 0;
+        mp.use_scale = 1;
+        mp.emit_pk = 1;
+    }
     extern __shared__ __align__(16) unsigned char smem[];
     double *sc = reinterpret_cast<double *>(smem);                     // [M]
     V *tw = reinterpret_cast<V *>(sc + M);                             // [max(1, M/2)]
     V *Z = tw + (M / 2 > 0 ? M / 2 : 1);                               // [R2][PITCH]
+    const uint64_t g = blockIdx.y;
     for (int i = threadIdx.x; i < M / 2; i += Q::T) {
+        const double2 z2 = tw_full[(uint64_t)i * mp.tw_step];
         V z;
-        z.x = (R)twiddles[i].x;
-        z.y = (R)twiddles[i].y;
+        z.x = (R)z2.x;
+        z.y = (R)z2.y;
         tw[i] = z;
     }
-    for (int i = threadIdx.x; i < M; i += Q::T) sc[i] = scale[i];
-    const R inv = (R)(1.0 / sqrt((double)M));
+    for (int i = threadIdx.x; i < M; i += Q::T) sc[i] = mp.use_scale ? scale[g * mp.gi + (uint64_t)i * mp.ji] : 1.0;
+    const R fo = (R)mp.f;
     double acc[Q::OWN], cmp[Q::OWN];
     static_for<0, Q::OWN>([&](auto i) { acc[i.value] = 0.0; cmp[i.value] = 0.0; });
     // tiles in an order that gives workgroups on one XCD (blockIdx mod 8) neighbouring columns
@@ -1209,8 +1235,9 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
     auto load_rows = [&](uint64_t tt, uint64_t Dt, int c, int j1) {
         const uint64_t d0 = tt * (uint64_t)W;
         const bool live = (uint64_t)c < Dt - d0;
-        const V *src = state + (uint64_t)j1 * Dt + d0 + (live ? c : 0);
-        static_for<0, R2>([&](auto q) { x[q.value] = src[(uint64_t)(R1 * q.value) * Dt]; });
+        const V *s0 = src + (g * mp.gi + (uint64_t)j1 * mp.ji) * Dt + d0 + (live ? c : 0);
+        const uint64_t step = (uint64_t)R1 * mp.ji * Dt;
+        static_for<0, R2>([&](auto q) { x[q.value] = s0[(uint64_t)q.value * step]; });
     };
     if (Q::PREFETCH && rank < ntiles && (int)threadIdx.x / W < R1) load_rows(rank, D, threadIdx.x % W, threadIdx.x / W);
     __syncthreads();
@@ -1255,41 +1282,58 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
         static_for<0, R1>([&](auto q) { u[q.value] = Z[k2 * Q::PITCH + q.value * W + c]; });
         reg_dft<Q::LR1>(u, tw, M);
         double p[R1];
-        V *dst = state + (uint64_t)k2 * Dt + d0 + c;
+        V *d1 = dst + (g * mp.go + (uint64_t)k2 * mp.ko) * Dt + d0 + c;
+        const uint64_t ostep = (uint64_t)R2 * mp.ko * Dt;
         static_for<0, R1>([&](auto q) {
             V y;
-            y.x = u[q.value].x * inv;
-            y.y = u[q.value].y * inv;
-            p[q.value] = live ? (double)y.x * (double)y.x + (double)y.y * (double)y.y : 0.0;
-            if (live) dst[(uint64_t)(R2 * q.value) * Dt] = y;
-        });
-        // sum over the W columns of the tile: a halving exchange across the lanes of the
-        // column group leaves each lane OWN of the R1 sums (every lane all of them when R1 < W)
-        static_for<0, Q::LW>([&](auto st) {
-            constexpr int mask = 1 << st.value;
-            constexpr int n = R1 >> st.value;      // values still held before this step
-            const bool hi = (c & mask) != 0;
-            if constexpr (n > 1) {
-                constexpr int h = n / 2;
-                static_for<0, h>([&](auto i) {
-                    const double lo_v = p[i.value], hi_v = p[h + i.value];
-                    const double send = hi ? lo_v : hi_v;
-                    const double keep = hi ? hi_v : lo_v;
-                    p[i.value] = keep + __shfl_xor(send, mask, W);
-                });
-            } else {
-                p[0] += __shfl_xor(p[0], mask, W);
+            y.x = u[q.value].x * fo;
+            y.y = u[q.value].y * fo;
+            if (SPLIT && mp.group_twiddle) {
+                // e^{-2 pi i g k / Mt} for output k = k2 + R2 q of this group (g k < Mt)
+                const uint64_t e = g * (uint64_t)(k2 + R2 * q.value);
+                double2 z2 = tw_full[e & (mp.Mt / 2 - 1)];
+                if (e >= mp.Mt / 2) {
+                    z2.x = -z2.x;
+                    z2.y = -z2.y;
+                }
+                V z;
+                z.x = (R)z2.x;
+                z.y = (R)z2.y;
+                y = cmul(y, z);
             }
+            p[q.value] = live ? (double)y.x * (double)y.x + (double)y.y * (double)y.y : 0.0;
+            if (live) d1[(uint64_t)q.value * ostep] = y;
         });
-        static_for<0, Q::OWN>([&](auto i) {
-            const double x = p[i.value], a = acc[i.value];
-            const double s2 = a + x;
-            if (fabs(a) >= fabs(x)) cmp[i.value] += (a - s2) + x;
-            else cmp[i.value] += (x - s2) + a;
-            acc[i.value] = s2;
-        });
+        if (mp.emit_pk) {
+            // sum over the W columns of the tile: a halving exchange across the lanes of the
+            // column group leaves each lane OWN of the R1 sums (every lane all of them when R1 < W)
+            static_for<0, Q::LW>([&](auto st) {
+                constexpr int mask = 1 << st.value;
+                constexpr int n = R1 >> st.value;      // values still held before this step
+                const bool hi = (c & mask) != 0;
+                if constexpr (n > 1) {
+                    constexpr int h = n / 2;
+                    static_for<0, h>([&](auto i) {
+                        const double lo_v = p[i.value], hi_v = p[h + i.value];
+                        const double send = hi ? lo_v : hi_v;
+                        const double keep = hi ? hi_v : lo_v;
+                        p[i.value] = keep + __shfl_xor(send, mask, W);
+                    });
+                } else {
+                    p[0] += __shfl_xor(p[0], mask, W);
+                }
+            });
+            static_for<0, Q::OWN>([&](auto i) {
+                const double x = p[i.value], a = acc[i.value];
+                const double s2 = a + x;
+                if (fabs(a) >= fabs(x)) cmp[i.value] += (a - s2) + x;
+                else cmp[i.value] += (x - s2) + a;
+                acc[i.value] = s2;
+            });
+        }
         __syncthreads();
     }
+    if (!mp.emit_pk) return;
     // the k1 values a lane holds: bit b of its column picks half b of what remained
     const int c = threadIdx.x % W, k2 = threadIdx.x / W;
     int base = 0, n = R1;
@@ -1304,7 +1348,8 @@ qift_reg_kernel(typename Vec<R>::type *state, uint64_t D, const double *scale, c
     }
     if (writer) {
         static_for<0, Q::OWN>([&](auto i) {
-            pk_partial[(uint64_t)blockIdx.x * M + (uint32_t)(k2 + R2 * (base + i.value))] = acc[i.value] + cmp[i.value];
+            const uint64_t k = (uint64_t)(k2 + R2 * (base + i.value));
+            pk_partial[(uint64_t)blockIdx.x * mp.Mt + g * mp.gk + k * mp.kk] = acc[i.value] + cmp[i.value];
         });
     }
 }
@@ -1958,37 +2003,103 @@ void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, d
     hip_check(hipGetLastError(), "block norm launch");
 }
 
-template <typename R, int LOGM>
-static bool launch_qift_reg(void *state, uint64_t D, const double *scale, const double *twiddles, double *pk_partial,
-                            int max_grid, size_t max_lds, int *grid_out, hipStream_t s) {
+// one launch of the register-form kernel with `groups` groups; max_rows caps the rows of
+// P(k) partial sums when the map emits them (*rows_out: rows written)
+template <typename R, int LOGM, bool SPLIT>
+static bool launch_qift_reg(const void *src, void *dst, uint64_t D, const QiftMap &mp, uint32_t groups,
+                            const double *scale, const double *twiddles, double *pk_partial, int max_rows,
+                            size_t max_lds, int *rows_out, hipStream_t s) {
     using Q = QiftShape<R, LOGM>;
     const size_t lds = Q::lds_bytes();
     if (lds > max_lds) return false;
     const uint64_t ntiles = (D + Q::W - 1) / Q::W;
-    int grid = (int)std::min<uint64_t>(ntiles, (uint64_t)std::min(1024, max_grid));
-    if (grid >= 8) grid &= ~7;
-    if (grid < 1) throw std::runtime_error("no workgroup fits the P(k) partial sums");
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&qift_reg_kernel<R, LOGM>),
+    // about 1024 workgroups in all, at most max_rows along x when P(k) rows are written
+    uint64_t gx = std::max<uint64_t>(1, 1024 / groups);
+    if (mp.emit_pk) gx = std::min<uint64_t>(gx, (uint64_t)max_rows);
+    gx = std::min<uint64_t>(gx, ntiles);
+    if (gx >= 8) gx &= ~7ull;
+    if (gx < 1) throw std::runtime_error("no workgroup fits the P(k) partial sums");
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&qift_reg_kernel<R, LOGM, SPLIT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((qift_reg_kernel<R, LOGM>), dim3(grid), dim3(Q::T), lds, s,
-                       reinterpret_cast<typename Vec<R>::type *>(state), D, scale,
+    hipLaunchKernelGGL((qift_reg_kernel<R, LOGM, SPLIT>), dim3((unsigned)gx, groups), dim3(Q::T), lds, s,
+                       reinterpret_cast<const typename Vec<R>::type *>(src),
+                       reinterpret_cast<typename Vec<R>::type *>(dst), D, mp, scale,
                        reinterpret_cast<const double2 *>(twiddles), pk_partial);
     hip_check(hipGetLastError(), "qift launch");
-    *grid_out = grid;
+    if (rows_out) *rows_out = (int)gx;
     return true;
 }
 
-template <typename R>
-static bool qift_reg(int m, void *state, uint64_t D, const double *scale, const double *twiddles, double *pk_partial,
-                     int max_grid, size_t max_lds, int *grid_out, hipStream_t s) {
-    switch (m) {
-#define QPE_QIFT_REG(L) \
-    case L: return launch_qift_reg<R, L>(state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s);
-        QPE_QIFT_REG(1) QPE_QIFT_REG(2) QPE_QIFT_REG(3) QPE_QIFT_REG(4) QPE_QIFT_REG(5)
-        QPE_QIFT_REG(6) QPE_QIFT_REG(7) QPE_QIFT_REG(8) QPE_QIFT_REG(9) QPE_QIFT_REG(10)
-#undef QPE_QIFT_REG
-    default: return false;
+template <typename R, bool SPLIT>
+static bool qift_reg_launch(int logr, const void *src, void *dst, uint64_t D, const QiftMap &mp, uint32_t groups,
+                            const double *scale, const double *twiddles, double *pk_partial, int max_rows,
+                            size_t max_lds, int *rows_out, hipStream_t s) {
+#define QPE_QIFT_REG(L)                                                                                         \
+    case L:                                                                                                     \
+        return launch_qift_reg<R, L, SPLIT>(src, dst, D, mp, groups, scale, twiddles, pk_partial, max_rows, max_lds, \
+                                            rows_out, s);
+    if constexpr (SPLIT) {
+        // the halves of 2^11 .. 2^20
+        switch (logr) {
+            QPE_QIFT_REG(5) QPE_QIFT_REG(6) QPE_QIFT_REG(7) QPE_QIFT_REG(8) QPE_QIFT_REG(9) QPE_QIFT_REG(10)
+        default: return false;
+        }
+    } else {
+        switch (logr) {
+            QPE_QIFT_REG(1) QPE_QIFT_REG(2) QPE_QIFT_REG(3) QPE_QIFT_REG(4) QPE_QIFT_REG(5)
+            QPE_QIFT_REG(6) QPE_QIFT_REG(7) QPE_QIFT_REG(8) QPE_QIFT_REG(9) QPE_QIFT_REG(10)
+        default: return false;
+        }
     }
+#undef QPE_QIFT_REG
+}
+
+// The register-form QIFT: one launch for 2^m <= 1024; for 2^m up to 2^20 the four-step
+// split 2^m = R1 * R2 (R1 = 2^floor(m/2)) as two launches: A, the R2-point DFTs of the rows
+// j1 + R1 j2 in place, times e^{-2 pi i j1 k2 / 2^m}; B, the R1-point DFTs of the rows
+// j1 + R1 k2 into `work` (a second register) in natural order, which is then copied back.
+template <typename R>
+static bool qift_reg(int m, void *state, void *work, uint64_t D, const double *scale, const double *twiddles,
+                     double *pk_partial, int max_rows, size_t max_lds, int *rows_out, hipStream_t s) {
+    const uint64_t M = 1ull << m;
+    QiftMap mp{};
+    mp.Mt = (uint32_t)M;
+    mp.f = 1.0;
+    if (m <= 10) {
+        mp.gi = 0; mp.ji = 1; mp.go = 0; mp.ko = 1; mp.gk = 0; mp.kk = 1;
+        mp.tw_step = 1;
+        mp.use_scale = 1;
+        mp.emit_pk = 1;
+        mp.f = 1.0 / std::sqrt((double)M);
+        return qift_reg_launch<R, false>(m, state, state, D, mp, 1, scale, twiddles, pk_partial, max_rows, max_lds,
+                                         rows_out, s);
+    }
+    if (m > 20) return false;
+    if (!work) throw std::runtime_error("the QIFT of 2^m > 1024 needs a second register's worth of device memory");
+    const int mb = m / 2, ma = m - mb;   // R1 = 2^mb groups of R2 = 2^ma points, then R2 groups of R1 points
+    const uint64_t R1 = 1ull << mb, R2 = 1ull << ma;
+    QiftMap a{};
+    a.gi = 1; a.ji = R1; a.go = 1; a.ko = R1;
+    a.Mt = (uint32_t)M;
+    a.tw_step = (uint32_t)(M / R2);
+    a.group_twiddle = 1;
+    a.use_scale = 1;
+    a.f = 1.0;
+    if (!qift_reg_launch<R, true>(ma, state, state, D, a, (uint32_t)R1, scale, twiddles, nullptr, max_rows, max_lds,
+                                  nullptr, s))
+        return false;
+    QiftMap b{};
+    b.gi = R1; b.ji = 1; b.go = 1; b.ko = R2; b.gk = 1; b.kk = R2;
+    b.Mt = (uint32_t)M;
+    b.tw_step = (uint32_t)(M / R1);
+    b.emit_pk = 1;
+    b.f = 1.0 / std::sqrt((double)M);
+    if (!qift_reg_launch<R, true>(mb, state, work, D, b, (uint32_t)R2, scale, twiddles, pk_partial, max_rows, max_lds,
+                                  rows_out, s))
+        throw std::runtime_error("QIFT second step did not launch");
+    hip_check(hipMemcpyAsync(state, work, M * D * sizeof(typename Vec<R>::type), hipMemcpyDeviceToDevice, s),
+              "QIFT copy back");
+    return true;
 }
 
 template <typename R, int LOGM>
@@ -2024,20 +2135,24 @@ static bool qift_mfma(int m, void *state, uint64_t D, const double *scale, const
 
 bool qift_form_available(int form, int m) {
     if (form == QIFT_MFMA) return m >= 4 && m <= 6;
-    if (form == QIFT_REGISTER) return m >= 1 && m <= 10;
+    if (form == QIFT_REGISTER) return m >= 1 && m <= 20;
+    if (form == QIFT_LDS_RADIX2) return m >= 1 && m <= 12;
     return true;
 }
 
+bool qift_needs_work(int form, int m) { return form != QIFT_MFMA && form != QIFT_LDS_RADIX2 && m > 10; }
+
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial, int *grid_out,
-          double *twiddles, size_t max_lds, hipStream_t s, int max_grid, int form) {
+          double *twiddles, size_t max_lds, hipStream_t s, int max_grid, int form, void *work) {
     if (form == QIFT_MFMA &&
         (dtype == 0 ? qift_mfma<double>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
                     : qift_mfma<float>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)))
         return;
-    // 2^m <= 1024: the register form; larger: radix-2 stages in LDS
+    // the register form (2^m <= 2^20); radix-2 stages in LDS when asked for
     if (form != QIFT_LDS_RADIX2 &&
-        (dtype == 0 ? qift_reg<double>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
-                    : qift_reg<float>(m, state, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)))
+        (dtype == 0
+             ? qift_reg<double>(m, state, work, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)
+             : qift_reg<float>(m, state, work, D, scale, twiddles, pk_partial, max_grid, max_lds, grid_out, s)))
         return;
     const uint32_t M = 1u << m;
     const size_t tw_bytes = (size_t)std::max<uint32_t>(1, M / 2) * sizeof(double2);
@@ -2103,6 +2218,8 @@ void qift_noalloc(int dtype, void *state, int m, uint64_t D, double *scale, doub
     if (M > kNoAllocScratchDoubles)
         throw std::runtime_error("2^m above the allocation-free QIFT's scratch (" +
                                  std::to_string(kNoAllocScratchDoubles) + " P(k) partial sums)");
+    if (qift_needs_work(form, m))
+        throw std::runtime_error("the allocation-free QIFT covers 2^m <= 1024 (larger ones need a second register)");
     double *part = scratch_address();
     int grid = 0;
     // every workgroup writes all 2^m of its partial sums, so the scratch needs no clearing

@@ -112,16 +112,18 @@ void evolve_in_place(const DevicePlan &dp, void *state, uint64_t block, uint64_t
 void block_norm(int dtype, uint64_t D, void *state, uint64_t j, double *norms, double *scale,
                 double *partials, int nparts, hipStream_t s);
 
-// QIFT kernels: 0 picks per size; the others exist for 2^m in 1..1024 (register DFTs),
-// 16..64 (matrix cores) and any size (radix-2 stages in LDS)
+// QIFT kernels: 0 picks per size; the others exist for 2^m in 2..2^20 (register DFTs),
+// 16..64 (matrix cores) and 2..4096 (radix-2 stages in LDS)
 enum QiftForm { QIFT_AUTO = 0, QIFT_REGISTER = 1, QIFT_MFMA = 2, QIFT_LDS_RADIX2 = 3 };
 bool qift_form_available(int form, int m);
+// whether qift() of this form and size needs `work` (a second register, 2^m * D amplitudes)
+bool qift_needs_work(int form, int m);
 
 // QIFT in place with the renormalisation factors applied, and P(k) partials.
 // max_grid caps the rows of 2^m partial sums written to pk_partial (*grid_out: rows written).
 void qift(int dtype, void *state, int m, uint64_t D, const double *scale, double *pk_partial,
           int *grid_out, double *twiddles, size_t max_lds, hipStream_t s, int max_grid = 1024,
-          int form = QIFT_AUTO);
+          int form = QIFT_AUTO, void *work = nullptr);
 
 // The same QIFT and P(k) with no allocation and no host transfer: the partial sums
 // in a static device array of kNoAllocScratchDoubles doubles (one per device), P(k)

@@ -238,7 +238,10 @@ int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *resu
 int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, double *results);
 
 /* StateVector::qift(), cpp/state_vector.hpp:115-179: in place, forward DFT
- * (e^{-2 pi i jk/M}) along the ancilla axis, natural order, divided by sqrt(M). */
+ * (e^{-2 pi i jk/M}) along the ancilla axis, natural order, divided by sqrt(M).
+ * For 2^precision > 1024 the default kernel needs a second register's worth of free
+ * device memory for the duration of the call (allocated and freed here); without it,
+ * 2^precision <= 4096 uses the radix-2 kernel and larger ones return QPE_ERR_OOM. */
 int qpe_qift(qpe_state_t s);
 
 /* StateVector::IdentifyPhaseAmplitudes(), cpp/state_vector.hpp:341-372:
@@ -252,7 +255,8 @@ int qpe_identify_phase_amplitudes(qpe_state_t s, double *probs);
  * caller-owned DEVICE memory of 2^precision doubles (QPE_ERR_ARG otherwise); it
  * receives P(k).  The per-workgroup P(k) partial sums live in a static device
  * array of 2^20 doubles (8 MiB per device, held until the process ends); states
- * of 2^precision > 2^20 outcomes are rejected (QPE_ERR_ARG).  Calls that overlap
+ * of 2^precision > 4096 outcomes are rejected (QPE_ERR_ARG: the QIFT kernels above
+ * that size need a second register).  Calls that overlap
  * on one device share that array and race: serialise them.  Afterwards the state
  * holds the QIFT output and its renormalisation factors are 1; a later
  * qpe_identify_phase_amplitudes computes P(k) from the register itself. */
@@ -284,9 +288,10 @@ enum qpe_option {
                                    call TEOT:136 makes); 1: applyUnitary (TEOT:174-372, the
                                    call left commented out at TEOT:135) */
     QPE_OPT_QIFT_FORM = 5       /* kernel of qpe_qift / qpe_qift_noalloc: 0 chosen by size
-                                   (default); 1 in-register DFTs (2^precision <= 1024);
-                                   2 DFT as a matrix product on the matrix cores
-                                   (2^precision 16..64); 3 radix-2 stages in LDS (any size).
+                                   (default); 1 in-register DFTs (one launch for
+                                   2^precision <= 1024, two up to 2^20); 2 DFT as a matrix
+                                   product on the matrix cores (2^precision 16..64);
+                                   3 radix-2 stages in LDS (2^precision <= 4096).
                                    All give the same QIFT within rounding. */
 };
 int qpe_state_set_option(qpe_state_t s, int option, int64_t value);

@@ -15,11 +15,11 @@ import numpy as np
 import pytest
 
 import qpesim
-from test_gpu_parity import rel_l2, say
+from test_gpu_parity import oracle_run, rel_l2, say
 
 pytestmark = pytest.mark.gpu
 
-FORMS = {qpesim.QIFT_REGISTER: range(1, 11), qpesim.QIFT_MFMA: range(4, 7), qpesim.QIFT_LDS_RADIX2: range(1, 11)}
+FORMS = {qpesim.QIFT_REGISTER: range(1, 13), qpesim.QIFT_MFMA: range(4, 7), qpesim.QIFT_LDS_RADIX2: range(1, 13)}
 
 
 def _case(m, n, e, dtype, form, seed):
@@ -51,6 +51,46 @@ def test_qift_forms_against_numpy(form, dtype):
             assert l2 < tol and dp < tol, (m, n, e, l2, dp)
             worst = (max(worst[0], l2), max(worst[1], dp))
     say(f"form {form} dtype {dtype}: worst L2 {worst[0]:.2e}, max|dP| {worst[1]:.2e}")
+
+
+@pytest.mark.parametrize("dtype", [qpesim.COMPLEX128, qpesim.COMPLEX64])
+def test_register_form_up_to_2_to_the_20(dtype):
+    """2^m > 1024: two launches of the register kernel (four-step split) and a second
+    register; 36 determinant columns so that numpy holds the register."""
+    tol = 1e-12 if dtype == qpesim.COMPLEX128 else 2e-6
+    for m in (13, 16, 17, 20):
+        l2, dp = _case(m, 4, 2, dtype, qpesim.QIFT_AUTO, 7 * m)
+        say(f"m={m} dtype {dtype}: L2 {l2:.2e}, max|dP| {dp:.2e}")
+        assert l2 < tol and dp < tol, (m, l2, dp)
+
+
+def test_large_m_qpe_against_oracle():
+    """A whole FP64 QPE at m = 13 (8191 Trotter steps of a small instance): the QIFT and
+    P(k) of the evolved register against numpy within 1e-12, the evolved register and
+    P(k) against the oracle's within 1e-11 (8191 steps accumulate the rounding
+    differences of the fused rotations against the oracle's per-term loop: measured
+    1.5e-12 on P(k))."""
+    n, e, m, seed = 4, 2, 13, 9
+    M = 1 << m
+    one, two = qpesim.synthetic_integrals(n, seed)
+    h, _ = qpesim.build_hamiltonian(n, e, one, two)
+    psi = qpesim.initial_state(n, e, seed, 0.8)
+    sv = qpesim.StateVector(m, n, e, qpesim.COMPLEX128)
+    sv.set(0, psi / math.sqrt(M))
+    qpesim.time_evolve_over_terms(sv, h)
+    X = sv.get().reshape(M, -1)
+    sv.qift()
+    Y = sv.get().reshape(M, -1)
+    P = sv.identify_phase_amplitudes()
+    sv.close()
+    Yr = np.fft.fft(X, axis=0) / math.sqrt(M)
+    assert rel_l2(Y, Yr) < 1e-12
+    assert float(np.max(np.abs(P - np.sum(np.abs(Yr) ** 2, axis=1)))) < 1e-12
+    ra, _, rP = oracle_run(n, e, m, seed, hf=0.8)
+    l2 = rel_l2(X.reshape(-1), ra.reshape(-1))
+    dp = float(np.max(np.abs(P - rP)))
+    say(f"m=13 QPE: evolved register L2 vs oracle {l2:.2e}, max|dP| vs oracle {dp:.2e}")
+    assert l2 < 1e-11 and dp < 1e-11
 
 
 def test_matrix_core_form_in_a_renormalised_qpe():

@@ -87,3 +87,28 @@ def test_qift_noalloc_refuses_host_memory():
     sv.qift()
     assert sv.identify_phase_amplitudes().shape == (8,)
     sv.close()
+
+
+def test_qift_noalloc_above_1024_outcomes():
+    """Up to 4096 outcomes the allocation-free QIFT takes the radix-2 kernel (the register
+    kernel would need a second register there); above that it is refused."""
+    M = 1 << 12
+    buf = DeviceBuffer(8 * M)
+    try:
+        sv = qpesim.StateVector(12, 4, 2, qpesim.COMPLEX128)
+        rng = np.random.default_rng(3)
+        X = (rng.normal(size=(M, 36)) + 1j * rng.normal(size=(M, 36))) / math.sqrt(M * 36)
+        sv.set(0, X.reshape(-1))
+        sv.qift_noalloc(buf.ptr.value)
+        Pr = np.sum(np.abs(np.fft.fft(X, axis=0) / math.sqrt(M)) ** 2, axis=1)
+        dp = float(np.max(np.abs(buf.to_host(M) - Pr)))
+        say(f"no-alloc QIFT 2^12 outcomes: max|dP| vs numpy {dp:.2e}")
+        assert dp < 1e-12
+        sv.close()
+        sv = qpesim.StateVector(13, 4, 2, qpesim.COMPLEX128)
+        with pytest.raises(qpesim.QpeError) as ei:
+            sv.qift_noalloc(buf.ptr.value)
+        assert ei.value.code == qpesim.ERR_ARG
+        sv.close()
+    finally:
+        buf.free()
