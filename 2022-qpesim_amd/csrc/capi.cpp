@@ -93,6 +93,8 @@ struct qpe_state {
     int order = 1;   // Trotter order
     int term_order = 0;        // QPE_OPT_TERM_ORDER of the chain
     int qift_form = 0;         // QPE_OPT_QIFT_FORM
+    int execution = 0;         // QPE_OPT_EXECUTION
+    uint32_t wait_ms = 10000;  // QPE_OPT_WAIT_LIMIT_MS
     int pk_rows = 0;           // rows of pk_partial
     int plan_term_order = -1;  // term order of the cached plan
     uint64_t evolved_to = 0;   // blocks < evolved_to hold U^j psi (chained qpe_time_evolve_blocks)
@@ -581,6 +583,14 @@ int qpe_state_set_option(qpe_state_t s, int option, int64_t value) {
                     "this QIFT form does not cover 2^precision (register DFTs: 2..1024; matrix cores: 16..64)");
             s->qift_form = (int)value;
             break;
+        case QPE_OPT_EXECUTION:
+            require(value >= 0 && value <= 2, QPE_ERR_ARG, "execution must be 0 (auto), 1 (passes) or 2 (dataflow)");
+            s->execution = (int)value;
+            break;
+        case QPE_OPT_WAIT_LIMIT_MS:
+            require(value >= 0 && value <= 3600000, QPE_ERR_ARG, "wait limit must be 0..3600000 ms");
+            s->wait_ms = (uint32_t)value;
+            break;
         case QPE_OPT_MAX_TILE_BYTES:
             require(value >= 0, QPE_ERR_ARG, "negative tile budget");
             s->max_tile_bytes = value;
@@ -630,6 +640,8 @@ static void finish_evolve(qpe_state_t s, float ms) {
     s->pk_valid = false;
     int status = 0;
     qpe::hip_check(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost), "status copy");
+    require(status != 2, QPE_ERR_STALLED,
+            "evolution stalled: a dataflow item waited longer than the wait limit for the items it depends on");
     require(status == 0, QPE_ERR_NUMERIC, "non-finite amplitudes during evolution");
 }
 
@@ -690,8 +702,10 @@ int qpe_time_evolve_blocks(qpe_state_t s, qpe_hamiltonian_t h, uint64_t j_begin,
             }
             s->launches = (int)s->plan->plan.passes.size() + 2;
         } else {
+            require(s->execution != qpe::EXEC_DATAFLOW || s->plan->flow_ok, QPE_ERR_ARG,
+                    "the dataflow evolution needs every pass tiled in LDS and blocks below 2^31 bytes");
             qpe::evolve(*s->plan, s->amp, j_begin, j_end, s->norms, s->scale, s->renorm, s->status, s->stream,
-                        &s->launches);
+                        &s->launches, s->execution, s->wait_ms);
         }
         qpe::hip_check(hipEventRecord(e1, s->stream), "event record");
         qpe::hip_check(hipEventSynchronize(e1), "evolve sync");

@@ -31,6 +31,15 @@ struct DevicePlan {
     double *partials = nullptr;   // [max_items] norm partial sums
     size_t lds_bytes = 0;         // largest dynamic LDS of a pass launch
     std::vector<size_t> pass_lds; // dynamic LDS of each pass launch
+    // dataflow order of a step (build_flow), for plans whose passes are all LDS-tiled
+    // and whose blocks are below 2^31 bytes
+    bool flow_ok = false;
+    uint32_t flow_items = 0;       // items per step
+    uint32_t flow_norm_items = 0;  // items of the pass that sums the block norm
+    uint32_t *flow_item_pass = nullptr, *flow_pass_first = nullptr, *flow_dep_off = nullptr, *flow_deps = nullptr;
+    PassDev *flow_passes = nullptr;
+    uint32_t *flow_done = nullptr; // [flow_items] completion stamps
+    uint32_t *flow_ctr = nullptr;  // [4] claim, norm count, finished step, abort
 
     void upload(const Plan &p, int dtype, hipStream_t s);
     void release();
@@ -99,8 +108,14 @@ struct BatchRun {
 // amplitudes): block j = U block j-1.  norms[M] and scale[M] are device arrays;
 // norms and scale of blocks below j_begin must be set.  renorm selects whether
 // each block is rescaled to block 0's norm when it is read.
+// execution: EXEC_AUTO runs the dataflow kernel when the plan allows it,
+// EXEC_PASSES one launch per pass, EXEC_DATAFLOW the dataflow kernel (throws
+// when the plan does not allow it).  flow_wait_ms bounds each wait of a
+// dataflow item; a launch whose wait runs out stops with *status_flag = 2.
+enum Execution { EXEC_AUTO = 0, EXEC_PASSES = 1, EXEC_DATAFLOW = 2 };
 void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end, double *norms, double *scale,
-            int renorm, int *status_flag, hipStream_t s, int *launches_per_step);
+            int renorm, int *status_flag, hipStream_t s, int *launches_per_step, int execution = EXEC_AUTO,
+            uint32_t flow_wait_ms = 10000);
 
 // time_evolve_iqpe (cpp/time_evolution_over_terms.hpp:141-155): the plan's
 // step applied `count` times in place to block `block`, without

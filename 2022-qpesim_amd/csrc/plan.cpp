@@ -840,4 +840,85 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
     return P;
 }
 
+Flow build_flow(const Plan &P) {
+    Flow F;
+    const uint32_t C = P.C;
+    if (P.passes.empty() || !(P.passes[0].flags & PF_FROM_PREV))
+        throw std::runtime_error("dataflow order: the first pass must read the previous block");
+    uint32_t nitems = 0;
+    for (const PassDev &ps : P.passes) {
+        F.pass_first.push_back(nitems);
+        nitems += (uint32_t)ps.item_count;
+    }
+    F.item_pass.assign(nitems, 0u);
+    F.dep_off.assign(nitems + 1, 0u);
+    // last writer of each column: a COL item (row_pass = false, id = item number), or
+    // every item of ROW pass id (row_pass = true)
+    struct Writer {
+        bool row_pass = false;
+        uint32_t id = 0;
+        bool set = false;
+    };
+    std::vector<Writer> last(C);
+    std::vector<uint32_t> mark(nitems, 0xffffffffu);   // dedup: the item whose list holds it
+    std::vector<uint32_t> list;
+    auto add = [&](uint32_t self, uint32_t d) {
+        if (mark[d] != self) {
+            mark[d] = self;
+            list.push_back(d);
+        }
+    };
+    auto overlaps = [](const ItemDev &a, const ItemDev &b) {
+        return a.begin < b.begin + b.count && b.begin < a.begin + a.count;
+    };
+    for (size_t k = 0; k < P.passes.size(); ++k) {
+        const PassDev &ps = P.passes[k];
+        if (k > 0 && (ps.flags & PF_FROM_PREV))
+            throw std::runtime_error("dataflow order: only the first pass may read the previous block");
+        const bool row = ps.kind == PASS_ROW;
+        for (int i = 0; i < ps.item_count; ++i) {
+            const uint32_t x = F.pass_first[k] + (uint32_t)i;
+            const ItemDev &it = P.items[ps.item_begin + i];
+            F.item_pass[x] = (uint32_t)k;
+            list.clear();
+            if (k > 0) {
+                auto follow = [&](const Writer &w) {
+                    if (!w.set) throw std::runtime_error("dataflow order: a column is read before it is written");
+                    if (!w.row_pass) {
+                        add(x, w.id);
+                        return;
+                    }
+                    const PassDev &wp = P.passes[w.id];
+                    for (int y = 0; y < wp.item_count; ++y)
+                        if (!row || overlaps(P.items[wp.item_begin + y], it)) add(x, F.pass_first[w.id] + (uint32_t)y);
+                };
+                if (row) {
+                    // a range of rows of every column; consecutive columns mostly share a writer
+                    for (uint32_t c = 0; c < C; ++c)
+                        if (c == 0 || last[c].row_pass != last[c - 1].row_pass || last[c].id != last[c - 1].id)
+                            follow(last[c]);
+                } else {
+                    for (uint32_t l = 0; l < it.count; ++l) follow(last[P.col_list[it.begin + l]]);
+                }
+            }
+            std::sort(list.begin(), list.end());
+            F.deps.insert(F.deps.end(), list.begin(), list.end());
+            F.dep_off[x + 1] = (uint32_t)F.deps.size();
+        }
+        // the pass's writes become the last ones (its items are disjoint)
+        if (row) {
+            for (uint32_t c = 0; c < C; ++c) last[c] = Writer{true, (uint32_t)k, true};
+        } else {
+            for (int i = 0; i < ps.item_count; ++i) {
+                const ItemDev &it = P.items[ps.item_begin + i];
+                for (uint32_t l = 0; l < it.count; ++l)
+                    last[P.col_list[it.begin + l]] = Writer{false, F.pass_first[k] + (uint32_t)i, true};
+            }
+        }
+    }
+    for (uint32_t c = 0; c < C; ++c)
+        if (!last[c].set) throw std::runtime_error("dataflow order: a column is never written");
+    return F;
+}
+
 }  // namespace qpe

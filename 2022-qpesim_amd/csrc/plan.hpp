@@ -181,4 +181,22 @@ struct Plan {
 Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool fuse = true, int order = 1,
                 int term_order = 0);
 
+// Dataflow order of one Trotter step: the items of every pass, numbered in pass
+// order (a second-order step's mirrored passes reuse the forward passes' item
+// ranges, so an item here is a (pass, item) pair), with, for each, the items of
+// earlier passes of the same step whose writes it reads.  A COL item reads and writes whole columns: it follows the
+// last writer of each of its columns (a COL item, or every item of a ROW pass).
+// A ROW item reads and writes a range of rows of every column: it follows the
+// last COL writer of every column and the items of a last ROW pass whose rows
+// overlap its own.  Items of the first pass (PF_FROM_PREV) read the previous
+// block only and follow the previous step as a whole.  Every dependency points
+// to an earlier item, so items claimed in index order can always finish.
+struct Flow {
+    std::vector<uint32_t> item_pass;   // [items] pass of each item
+    std::vector<uint32_t> pass_first;  // [passes] number of the pass's first item
+    std::vector<uint32_t> dep_off;     // [items + 1] offsets into deps
+    std::vector<uint32_t> deps;        // item numbers
+};
+Flow build_flow(const Plan &P);
+
 }  // namespace qpe

@@ -24,8 +24,10 @@ COMPLEX128, COMPLEX64 = 0, 1
 LAYOUT_ALPHA_BETA, LAYOUT_SINGLE_STRING = 0, 1
 PP, PQ, PQQP, PQQR, PQRS = range(5)
 OPT_RENORMALIZE, OPT_ENGINE, OPT_MAX_TILE_BYTES, OPT_TROTTER_ORDER, OPT_TERM_ORDER, OPT_QIFT_FORM = 0, 1, 2, 3, 4, 5
+OPT_EXECUTION, OPT_WAIT_LIMIT_MS = 6, 7
+EXEC_AUTO, EXEC_PASSES, EXEC_DATAFLOW = 0, 1, 2
 QIFT_AUTO, QIFT_REGISTER, QIFT_MFMA, QIFT_LDS_RADIX2 = 0, 1, 2, 3
-ERR_ARG, ERR_OOM, ERR_HIP, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
+ERR_ARG, ERR_OOM, ERR_HIP, ERR_NUMERIC, ERR_STATE, ERR_STALLED = 1, 2, 3, 4, 5, 6
 
 # every entry point include/qpesim.h declares
 EXPORTS = [

@@ -32,7 +32,9 @@ enum qpe_status {
     QPE_ERR_OOM = 2,      /* request larger than the free device memory */
     QPE_ERR_HIP = 3,      /* a HIP runtime call failed */
     QPE_ERR_NUMERIC = 4,  /* non-finite amplitudes detected */
-    QPE_ERR_STATE = 5     /* call out of sequence (e.g. evolve before IndexTerms) */
+    QPE_ERR_STATE = 5,    /* call out of sequence (e.g. evolve before IndexTerms) */
+    QPE_ERR_STALLED = 6   /* a wait of the dataflow evolution exceeded QPE_OPT_WAIT_LIMIT_MS;
+                             the launch stopped and the blocks of that call are invalid */
 };
 
 enum qpe_dtype {
@@ -287,12 +289,22 @@ enum qpe_option {
                                    0: applyUnitary_reordered (TEOT:374-562; default, the
                                    call TEOT:136 makes); 1: applyUnitary (TEOT:174-372, the
                                    call left commented out at TEOT:135) */
-    QPE_OPT_QIFT_FORM = 5       /* kernel of qpe_qift / qpe_qift_noalloc: 0 chosen by size
+    QPE_OPT_QIFT_FORM = 5,      /* kernel of qpe_qift / qpe_qift_noalloc: 0 chosen by size
                                    (default); 1 in-register DFTs (one launch for
                                    2^precision <= 1024, two up to 2^20); 2 DFT as a matrix
                                    product on the matrix cores (2^precision 16..64);
                                    3 radix-2 stages in LDS (2^precision <= 4096).
                                    All give the same QIFT within rounding. */
+    QPE_OPT_EXECUTION = 6,      /* how qpe_time_evolve_over_terms / _blocks run the passes:
+                                   1: one launch per pass; 2: one dataflow launch for all
+                                   the steps of the call, its items starting as soon as the
+                                   items they read from are done (QPE_ERR_ARG unless every
+                                   pass is tiled in LDS and blocks are below 2^31 bytes);
+                                   0: 2 for complex128 when the plan allows it, else 1
+                                   (default).  Results are identical. */
+    QPE_OPT_WAIT_LIMIT_MS = 7   /* limit of one wait of a dataflow item for the items it
+                                   depends on (default 10000 ms; 0: no waiting, so a run
+                                   stops at its first wait); QPE_ERR_STALLED past it */
 };
 int qpe_state_set_option(qpe_state_t s, int option, int64_t value);
 
