@@ -763,12 +763,19 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
     }
 }
 
+struct NoGate {
+    __device__ bool operator()() const { return true; }
+};
+
 // One tile item of one pass: load the item's lines, run the pass's ops, write back.
 // FLOW: an item of the dataflow evolution (block traffic, the renormalisation
 // factor and the norm partial coherent at device scope; LDS-tiled passes only).
-template <typename R, int MODE, bool FLOW = false>
-__device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass, uint32_t item,
-                                          typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
+// gate() runs, in every thread, after the item's tables and first op are staged
+// and before its tile is read; false abandons the item (returns false).
+template <typename R, int MODE, bool FLOW = false, class Gate = NoGate>
+__device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass, uint32_t item,
+                                          typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j,
+                                          Gate gate = Gate()) {
     static_assert(!FLOW || MODE == 1, "dataflow items are LDS-tiled");
     using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -812,6 +819,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     }
     EntSet E0, E1;
     load_entries(A, ring[0], o0 < o1, E0);   // overlaps the tile load below
+    if (!gate()) return false;
 
     // ---- load the tile ---------------------------------------------------
     const bool from_prev = pass.flags & PF_FROM_PREV;
@@ -877,6 +885,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const PassDev &pass
     } else {
         copy_rows_out<V, FLOW>(cur, tile, it.begin, nl, C, blk_bytes);
     }
+    return true;
 }
 
 // MODE 1: LDS-tiled passes only, MODE 2: global-memory windows only (separate
@@ -1130,13 +1139,20 @@ __global__ void __launch_bounds__(kThreads) flow_kernel(PassArgs A, FlowArgs F) 
         const uint32_t j = F.j_begin + t / F.ipS, x = t % F.ipS;
         const uint32_t k = __builtin_amdgcn_readfirstlane(F.item_pass[x]);
         const PassDev ps = F.passes[k];
-        if (tid < 64) {
-            const bool ok = flow_wait(F, x, j, (ps.flags & PF_FROM_PREV) != 0);
-            if (tid == 0) bc[1] = ok ? 1u : 0u;
-        }
-        __syncthreads();
-        if (!bc[1]) break;
-        pass_body<R, 1, true>(A, ps, x - F.pass_first[k], base + (uint64_t)(j - 1) * A.D, base + (uint64_t)j * A.D, j);
+        // the wait comes after the item's set-up (line tables, first op batch and entries,
+        // all read-only), so their loads overlap it
+        const bool from_prev = (ps.flags & PF_FROM_PREV) != 0;
+        auto gate = [&]() -> bool {
+            if (tid < 64) {
+                const bool ok = flow_wait(F, x, j, from_prev);
+                if (tid == 0) bc[1] = ok ? 1u : 0u;
+            }
+            __syncthreads();
+            return bc[1] != 0u;
+        };
+        if (!pass_body<R, 1, true>(A, ps, x - F.pass_first[k], base + (uint64_t)(j - 1) * A.D,
+                                   base + (uint64_t)j * A.D, j, gate))
+            break;
         // every thread's block stores and the norm partial have completed before the stamp
         __builtin_amdgcn_s_waitcnt(kVmcntZero);
         __syncthreads();
@@ -2269,8 +2285,8 @@ void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end,
         throw std::runtime_error("the dataflow evolution needs LDS-tiled passes and blocks below 2^31 bytes");
     // auto: the dataflow kernel for complex128, whose tiles are half as wide (twice the
     // items per pass, so more passes overlap); complex64 measured faster pass by pass
-    // (cfg 32 spin-orbitals / 8 electrons: 61.7 against 70.8 ms per complex128 step,
-    // 39.8 against 37.1 ms per complex64 step)
+    // (32 spin-orbitals / 8 electrons: 60.0 against 70.7 ms per complex128 step, 38.3
+    // against 37.1 ms per complex64 step)
     const bool flow = execution == EXEC_DATAFLOW || (execution == EXEC_AUTO && dp.dtype == 0);
     if (flow && dp.flow_ok) {
         if (dp.dtype == 0)
