@@ -749,7 +749,7 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
                           double *elapsed_ms) {
     return guarded([&] {
         require(states != nullptr && hams != nullptr && count >= 1, QPE_ERR_ARG, "empty batch");
-        require((flags & ~QPE_BATCH_EAGER) == 0, QPE_ERR_ARG, "unknown batch flags");
+        require((flags & ~(QPE_BATCH_EAGER | QPE_BATCH_PASSES)) == 0, QPE_ERR_ARG, "unknown batch flags");
         for (int i = 0; i < count; ++i) {
             check_state(states[i]);
             for (int k = 0; k < i; ++k) require(states[k] != states[i], QPE_ERR_ARG, "a state appears twice in the batch");
@@ -766,6 +766,8 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
             key.push_back(states[i]->plan_id);
             key.push_back((uint64_t)states[i]->renorm);
         }
+        const bool use_flow = (flags & QPE_BATCH_PASSES) == 0;
+        key.push_back(use_flow ? 1u : 0u);
         if (!g_batch.run || g_batch.key != key) {
             std::vector<qpe::BatchMember> mem;
             for (int i = 0; i < count; ++i) {
@@ -780,7 +782,7 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
             }
             g_batch.run.reset(new qpe::BatchRun());
             g_batch.key.clear();
-            g_batch.run->build(mem, states[0]->M, states[0]->stream);
+            g_batch.run->build(mem, states[0]->M, states[0]->stream, use_flow);
             g_batch.key = key;
         }
         hipStream_t st = states[0]->stream;
@@ -799,7 +801,7 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
         (void)hipEventDestroy(e1);
         if (elapsed_ms) *elapsed_ms = ms;
         for (int i = 0; i < count; ++i) {
-            states[i]->launches = (int)g_batch.run->launches.size() + 1;
+            states[i]->launches = g_batch.run->flow ? 1 : (int)g_batch.run->launches.size() + 1;
             finish_evolve(states[i], ms);
             states[i]->evolved_to = states[i]->M;
         }
@@ -819,8 +821,10 @@ struct TuneCandidate {
 };
 
 // Times one Trotter step of every candidate tile width (lines per workgroup, from the widest
-// that fits down to 1) and grouping of instance (s, h) on a two-block scratch state.
-std::vector<TuneCandidate> tune_candidates(qpe_state_t s, qpe_hamiltonian_t h) {
+// that fits down to 1) and grouping of instance (s, h) on a two-block scratch state, with
+// the given execution (QPE_OPT_EXECUTION; a candidate the dataflow kernel cannot run is
+// timed pass by pass).
+std::vector<TuneCandidate> tune_candidates(qpe_state_t s, qpe_hamiltonian_t h, int execution) {
     check_state(s);
     check_ham(h);
     require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
@@ -841,11 +845,15 @@ std::vector<TuneCandidate> tune_candidates(qpe_state_t s, qpe_hamiltonian_t h) {
             t->engine = engine;
             t->order = s->order;
             t->term_order = s->term_order;
+            t->execution = execution;
             t->plan_version = ~0ull;
             t->plan.reset();
             double us = 0.0;
             bool ok = true;
             for (int rep = 0; rep < 3 && ok; ++rep) {   // first run builds the plan
+                if (rep == 0 && t->execution == qpe::EXEC_DATAFLOW && qpe_state_set(t, 0, 1, one) == QPE_OK &&
+                    qpe_time_evolve_over_terms(t, h) == QPE_ERR_ARG)
+                    t->execution = qpe::EXEC_PASSES;   // global-memory windows: no dataflow form
                 if (qpe_state_set(t, 0, 1, one) != QPE_OK || qpe_time_evolve_over_terms(t, h) != QPE_OK) ok = false;
                 else if (rep > 0) us += t->t_evolve * 1e3 / 2.0;
             }
@@ -879,7 +887,7 @@ void apply_choice(qpe_state_t s, const TuneCandidate &c, double *result) {
 int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *result) {
     return guarded([&] {
         require(share_cus >= 1, QPE_ERR_ARG, "share_cus must be positive");
-        const std::vector<TuneCandidate> cands = tune_candidates(s, h);
+        const std::vector<TuneCandidate> cands = tune_candidates(s, h, s->execution);
         size_t best = 0;
         double best_cost = 1e300;
         for (size_t i = 0; i < cands.size(); ++i) {
@@ -897,7 +905,8 @@ int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams,
     return guarded([&] {
         require(states != nullptr && hams != nullptr && count >= 1, QPE_ERR_ARG, "empty batch");
         std::vector<std::vector<TuneCandidate>> cands;
-        for (int i = 0; i < count; ++i) cands.push_back(tune_candidates(states[i], hams[i]));
+        // an instance of a dataflow batch runs its own chain: its candidates are timed that way
+        for (int i = 0; i < count; ++i) cands.push_back(tune_candidates(states[i], hams[i], qpe::EXEC_DATAFLOW));
         // fastest candidate of instance i with at most max_passes passes per step (none: the fewest passes)
         auto fastest = [&](int i, int max_passes) {
             size_t best = 0;
@@ -913,11 +922,31 @@ int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams,
                     if (cands[i][k].passes < cands[i][best].passes) best = k;
             return best;
         };
-        // Two policies, each timed as one batched step on two-block scratch states:
+        // Three policies, each timed as one batched step on two-block scratch states:
         //  A  every instance at its fewest passes (the fastest of those): the fewest launches;
         //  B  the instance whose fastest step is slowest at that fastest candidate, every other
-        //     instance at its fastest candidate with no more passes than that one.
-        std::vector<size_t> pol_a(count), pol_b(count);
+        //     instance at its fastest candidate with no more passes than that one;
+        //  C  every instance at its fastest candidate (the dataflow batch runs each instance's
+        //     chain at its own pace);
+        //  D  every instance at half its widest tile (fused groups): twice the items per pass,
+        //     so the instances of a dataflow batch share the CUs in smaller pieces (measured at
+        //     48 instances, 2^8 outcomes: 3.24 s complex64 and 2.97 s complex128 against 3.65 s
+        //     and 3.39 s with the widest tiles).
+        std::vector<size_t> pol_a(count), pol_b(count), pol_c(count), pol_d(count);
+        for (int i = 0; i < count; ++i) {
+            pol_c[i] = fastest(i, INT32_MAX);
+            int64_t widest = 0;
+            for (const auto &c : cands[i]) widest = std::max(widest, c.budget);
+            size_t pick = pol_c[i];
+            bool found = false;
+            for (size_t k = 0; k < cands[i].size(); ++k) {
+                const TuneCandidate &c = cands[i][k];
+                if (c.engine != 0 || c.budget > widest / 2 + 64) continue;
+                if (!found || c.budget > cands[i][pick].budget) pick = k;
+                found = true;
+            }
+            pol_d[i] = pick;
+        }
         for (int i = 0; i < count; ++i) {
             int minp = INT32_MAX;
             for (const auto &c : cands[i]) minp = std::min(minp, c.passes);
@@ -943,7 +972,8 @@ int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams,
             try {
                 const double one[2] = {1.0, 0.0};
                 for (int i = 0; i < count; ++i) {
-                    int rc = qpe_state_create(&t[i], 1, states[i]->n, states[i]->e, states[i]->dtype);
+                    // three steps per timed batch, so that steps overlap as in a long chain
+                    int rc = qpe_state_create(&t[i], 2, states[i]->n, states[i]->e, states[i]->dtype);
                     if (rc != QPE_OK) throw ApiError(rc, "autotune scratch state: " + std::string(qpe_last_error()));
                     t[i]->max_tile_bytes = cands[i][pol[i]].budget;
                     t[i]->engine = cands[i][pol[i]].engine;
@@ -970,8 +1000,14 @@ int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams,
         };
         const double ms_a = time_policy(pol_a);
         const double ms_b = pol_b == pol_a ? ms_a : time_policy(pol_b);
-        const std::vector<size_t> &pol = ms_b < ms_a ? pol_b : pol_a;
-        for (int i = 0; i < count; ++i) apply_choice(states[i], cands[i][pol[i]], results ? results + 3 * i : nullptr);
+        const double ms_c = pol_c == pol_a ? ms_a : pol_c == pol_b ? ms_b : time_policy(pol_c);
+        const double ms_d = time_policy(pol_d);
+        const std::vector<size_t> *pol = &pol_a;
+        double best = ms_a;
+        if (ms_b < best) { best = ms_b; pol = &pol_b; }
+        if (ms_c < best) { best = ms_c; pol = &pol_c; }
+        if (ms_d < best) { best = ms_d; pol = &pol_d; }
+        for (int i = 0; i < count; ++i) apply_choice(states[i], cands[i][(*pol)[i]], results ? results + 3 * i : nullptr);
     });
 }
 

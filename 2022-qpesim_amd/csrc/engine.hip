@@ -1041,7 +1041,8 @@ struct FlowArgs {
     const uint32_t *deps;        // item numbers
     const PassDev *passes;
     uint32_t *done;              // [ipS] step number of the item's last completion (0: none in this launch)
-    uint32_t *ctr;               // [0] next claim, [1] norm items done, [2] last finished step, [3] abort
+    uint32_t *ctr;               // this register's counters: [1] norm items done, [2] last finished step
+    uint32_t *gctr;              // the launch's counters: [0] next claim, [3] abort (one register: ctr)
     double *norms, *scale;
     int *status;
     uint32_t j_begin, j_end;
@@ -1073,9 +1074,9 @@ __device__ bool flow_wait(const FlowArgs &F, uint32_t x, uint32_t j, bool from_p
         if (step_wait && coh_load_u32(F.ctr + 2) + 1u >= j) step_wait = false;
         while (i < d1 && coh_load_u32(F.done + F.deps[i]) >= j) i += 64;
         if (__ballot(i < d1 || step_wait) == 0ull) return true;
-        if (coh_load_u32(F.ctr + 3)) return false;
+        if (coh_load_u32(F.gctr + 3)) return false;
         if (__builtin_amdgcn_s_memrealtime() - t0 > F.wait_ticks) {
-            coh_store_u32(F.ctr + 3, 1u);
+            coh_store_u32(F.gctr + 3, 1u);
             __hip_atomic_store(F.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -1116,61 +1117,99 @@ __device__ void flow_finalize(const PassArgs &A, const FlowArgs &F, uint32_t j, 
     __syncthreads();
 }
 
+// Item x of step j of one register: wait for the items it reads from (after its
+// read-only set-up, so their loads overlap the wait), run it, stamp it done, and
+// finish the step's norm when it completes the step's last norm item.  false:
+// the launch aborted.  bc: LDS broadcast words beside the pair count in the header
+// ([1] go, [2] finalize).
+template <typename R>
+__device__ __forceinline__ bool flow_item(const PassArgs &A, const FlowArgs &F, uint32_t x, uint32_t j,
+                                          volatile uint32_t *bc, unsigned char *smem) {
+    using V = typename Vec<R>::type;
+    const int tid = threadIdx.x;
+    V *base = reinterpret_cast<V *>(A.state);
+    const uint32_t k = __builtin_amdgcn_readfirstlane(F.item_pass[x]);
+    const PassDev ps = F.passes[k];
+    const bool from_prev = (ps.flags & PF_FROM_PREV) != 0;
+    auto gate = [&]() -> bool {
+        if (tid < 64) {
+            const bool ok = flow_wait(F, x, j, from_prev);
+            if (tid == 0) bc[1] = ok ? 1u : 0u;
+        }
+        __syncthreads();
+        return bc[1] != 0u;
+    };
+    if (!pass_body<R, 1, true>(A, ps, x - F.pass_first[k], base + (uint64_t)(j - 1) * A.D, base + (uint64_t)j * A.D,
+                               j, gate))
+        return false;
+    // every thread's block stores and the norm partial have completed before the stamp
+    __builtin_amdgcn_s_waitcnt(kVmcntZero);
+    __syncthreads();
+    if (tid == 0) {
+        coh_store_u32(F.done + x, j);
+        uint32_t fin = 0u;
+        if (ps.flags & PF_EMIT_NORM) {
+            const uint32_t c = __hip_atomic_fetch_add(F.ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            fin = c == (j - F.j_begin + 1u) * F.norm_items ? 1u : 0u;
+        }
+        bc[2] = fin;
+    }
+    __syncthreads();
+    if (bc[2]) flow_finalize(A, F, j, reinterpret_cast<double *>(smem + 512));
+    return true;
+}
+
+// the next claim of the launch (wave-uniform; ~0u once the launch aborted), through
+// the LDS broadcast word bc[0]
+__device__ inline uint32_t flow_claim(uint32_t *gctr, volatile uint32_t *bc) {
+    if (threadIdx.x == 0) {
+        uint32_t t = 0xffffffffu;
+        if (!coh_load_u32(gctr + 3)) t = __hip_atomic_fetch_add(gctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bc[0] = t;
+    }
+    __syncthreads();
+    // wave-uniform: read into a scalar register, so that the item's pass descriptor and
+    // everything derived from it stay scalar as in pass_kernel
+    return __builtin_amdgcn_readfirstlane(bc[0]);
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kThreads) flow_kernel(PassArgs A, FlowArgs F) {
-    using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    // broadcast words beside the pair count in the LDS header: [0] claim, [1] go, [2] finalize
     volatile uint32_t *bc = reinterpret_cast<volatile uint32_t *>(smem + 1028);
-    const int tid = threadIdx.x;
     const uint32_t total = (F.j_end - F.j_begin) * F.ipS;
-    V *base = reinterpret_cast<V *>(A.state);
     for (;;) {
-        if (tid == 0) {
-            uint32_t t = 0xffffffffu;
-            if (!coh_load_u32(F.ctr + 3)) t = __hip_atomic_fetch_add(F.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bc[0] = t;
-        }
-        __syncthreads();
-        // the claim is wave-uniform: read it into a scalar register, so that the item's
-        // pass descriptor and everything derived from it stay scalar as in pass_kernel
-        const uint32_t t = __builtin_amdgcn_readfirstlane(bc[0]);
+        const uint32_t t = flow_claim(F.gctr, bc);
         if (t >= total) break;
-        const uint32_t j = F.j_begin + t / F.ipS, x = t % F.ipS;
-        const uint32_t k = __builtin_amdgcn_readfirstlane(F.item_pass[x]);
-        const PassDev ps = F.passes[k];
-        // the wait comes after the item's set-up (line tables, first op batch and entries,
-        // all read-only), so their loads overlap it
-        const bool from_prev = (ps.flags & PF_FROM_PREV) != 0;
-        auto gate = [&]() -> bool {
-            if (tid < 64) {
-                const bool ok = flow_wait(F, x, j, from_prev);
-                if (tid == 0) bc[1] = ok ? 1u : 0u;
-            }
-            __syncthreads();
-            return bc[1] != 0u;
-        };
-        if (!pass_body<R, 1, true>(A, ps, x - F.pass_first[k], base + (uint64_t)(j - 1) * A.D,
-                                   base + (uint64_t)j * A.D, j, gate))
-            break;
-        // every thread's block stores and the norm partial have completed before the stamp
-        __builtin_amdgcn_s_waitcnt(kVmcntZero);
-        __syncthreads();
-        if (tid == 0) {
-            coh_store_u32(F.done + x, j);
-            uint32_t fin = 0u;
-            if (ps.flags & PF_EMIT_NORM) {
-                const uint32_t c = __hip_atomic_fetch_add(F.ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-                fin = c == (j - F.j_begin + 1u) * F.norm_items ? 1u : 0u;
-            }
-            bc[2] = fin;
-        }
-        __syncthreads();
-        if (bc[2]) flow_finalize(A, F, j, reinterpret_cast<double *>(smem + 512));
+        if (!flow_item<R>(A, F, t % F.ipS, F.j_begin + t / F.ipS, bc, smem)) break;
+    }
+}
+
+// Batched form: the steps 1..steps of several registers in one launch.  Claim t
+// is step 1 + t / per_step, member and item order[t % per_step]: one step of every
+// member, the members' items interleaved (each member's in its own order, which is
+// all that deadlock freedom needs: a member's dependencies are earlier claims).
+template <typename R>
+__global__ void __launch_bounds__(kThreads) flow_batch_kernel(const PassArgs *__restrict__ args,
+                                                              const FlowArgs *__restrict__ flows,
+                                                              const uint2 *__restrict__ order, uint32_t per_step,
+                                                              uint32_t steps, uint32_t *gctr) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    volatile uint32_t *bc = reinterpret_cast<volatile uint32_t *>(smem + 1028);
+    const uint32_t total = steps * per_step;
+    for (;;) {
+        const uint32_t t = flow_claim(gctr, bc);
+        if (t >= total) break;
+        const uint2 mi = order[t % per_step];
+        const uint32_t inst = __builtin_amdgcn_readfirstlane(mi.x), x = __builtin_amdgcn_readfirstlane(mi.y);
+        const PassArgs A = from_global_args(reinterpret_cast<const PassArgsG *>(args)[inst]);
+        const FlowArgs F = flows[inst];
+        if (!flow_item<R>(A, F, x, 1u + t / per_step, bc, smem)) break;
     }
 }
 
 // before a dataflow launch: no item done, no claim, steps below j_begin finished
+// (ctr: the register's counters; the launch's, gctr, too when it is the same array)
 __global__ void flow_reset(uint32_t *done, uint32_t n, uint32_t *ctr, uint32_t finished) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) done[i] = 0u;
@@ -2074,6 +2113,7 @@ static void evolve_flow_t(const DevicePlan &dp, void *state, uint64_t j_begin, u
     F.passes = dp.flow_passes;
     F.done = dp.flow_done;
     F.ctr = dp.flow_ctr;
+    F.gctr = dp.flow_ctr;
     F.norms = norms;
     F.scale = scale;
     F.status = status_flag;
@@ -2125,7 +2165,7 @@ static void evolve_in_place_t(const DevicePlan &dp, void *state, uint64_t block,
 
 // ---- batched evolution ---------------------------------------------------------
 
-void BatchRun::build(const std::vector<BatchMember> &members, uint64_t M_, hipStream_t s) {
+void BatchRun::build(const std::vector<BatchMember> &members, uint64_t M_, hipStream_t s, bool use_flow) {
     release();
     if (members.empty()) throw std::runtime_error("empty batch");
     count = (int)members.size();
@@ -2183,7 +2223,89 @@ void BatchRun::build(const std::vector<BatchMember> &members, uint64_t M_, hipSt
     to_device(&this->fin, fin, s);
     hip_check(hipMalloc(reinterpret_cast<void **>(&jstep), sizeof(uint64_t)), "hipMalloc(step counter)");
     hip_check(hipMalloc(reinterpret_cast<void **>(&done), sizeof(unsigned)), "hipMalloc(done counter)");
+    // dataflow form: every member's plan has its dependency tables
+    uint64_t per_step = 0;
+    bool eligible = use_flow;
+    for (const auto &m : members) {
+        eligible = eligible && m.dp->flow_ok;
+        per_step += m.dp->flow_items;
+    }
+    if (eligible && per_step > 0 && (M - 1) * per_step < (1ull << 31)) {
+        // one step's claims: member i's items at positions (x + 1/2) / items_i, merged in
+        // position order (ties by member), so every member advances through its step at
+        // the same relative pace
+        std::vector<std::pair<double, uint64_t>> pos;
+        pos.reserve(per_step);
+        for (int i = 0; i < count; ++i) {
+            const uint32_t n = members[i].dp->flow_items;
+            for (uint32_t x = 0; x < n; ++x) pos.push_back({(x + 0.5) / n, ((uint64_t)i << 32) | x});
+        }
+        std::stable_sort(pos.begin(), pos.end(),
+                         [](const std::pair<double, uint64_t> &a, const std::pair<double, uint64_t> &b) {
+                             return a.first < b.first;
+                         });
+        std::vector<uint32_t> ord;
+        ord.reserve(2 * per_step);
+        for (const auto &p : pos) {
+            ord.push_back((uint32_t)(p.second >> 32));
+            ord.push_back((uint32_t)p.second);
+        }
+        std::vector<FlowArgs> fa;
+        for (int i = 0; i < count; ++i) {
+            const DevicePlan &dp = *members[i].dp;
+            FlowArgs F;
+            F.item_pass = dp.flow_item_pass;
+            F.pass_first = dp.flow_pass_first;
+            F.dep_off = dp.flow_dep_off;
+            F.deps = dp.flow_deps;
+            F.passes = dp.flow_passes;
+            F.done = dp.flow_done;
+            F.ctr = dp.flow_ctr;
+            F.gctr = nullptr;   // set below
+            F.norms = members[i].norms;
+            F.scale = members[i].scale;
+            F.status = members[i].status;
+            F.j_begin = 1;
+            F.j_end = (uint32_t)M;
+            F.ipS = dp.flow_items;
+            F.norm_items = dp.flow_norm_items;
+            F.wait_ticks = 10000ull * 100000ull;   // 10 s
+            fa.push_back(F);
+            flow_plans.push_back(&dp);
+        }
+        hip_check(hipMalloc(reinterpret_cast<void **>(&flow_gctr), 4 * sizeof(uint32_t)), "hipMalloc(batch counters)");
+        for (auto &F : fa) F.gctr = flow_gctr;
+        FlowArgs *fd = nullptr;
+        to_device(&fd, fa, s);
+        flow_args = fd;
+        uint32_t *od = nullptr;
+        to_device(&od, ord, s);
+        flow_order = od;
+        flow_per_step = (uint32_t)per_step;
+        flow = true;
+    }
     hip_check(hipStreamSynchronize(s), "batch upload sync");
+}
+
+// the dataflow form's launch sequence: every member's stamps and counters reset, the
+// launch counters reset, one persistent launch for steps 1..M-1
+template <typename R>
+static void batch_flow_launch(const BatchRun &b, hipStream_t s) {
+    for (const DevicePlan *dp : b.flow_plans)
+        hipLaunchKernelGGL(flow_reset, dim3((dp->flow_items + 255) / 256), dim3(256), 0, s, dp->flow_done,
+                           dp->flow_items, dp->flow_ctr, 0u);
+    hipLaunchKernelGGL(flow_reset, dim3(1), dim3(64), 0, s, (uint32_t *)nullptr, 0u, b.flow_gctr, 0u);
+    const void *fn = reinterpret_cast<const void *>(&flow_batch_kernel<R>);
+    int dev = 0, cus = 0, per_cu = 0;
+    hip_check(hipGetDevice(&dev), "device");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, b.lds_bytes), "flow occupancy");
+    if (per_cu < 1) throw std::runtime_error("dataflow kernel does not fit a compute unit");
+    const uint64_t claims = (b.M - 1) * b.flow_per_step;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(claims, (uint64_t)cus * per_cu);
+    hipLaunchKernelGGL((flow_batch_kernel<R>), dim3(grid), dim3(kThreads), b.lds_bytes, s,
+                       reinterpret_cast<const PassArgs *>(b.args), reinterpret_cast<const FlowArgs *>(b.flow_args),
+                       reinterpret_cast<const uint2 *>(b.flow_order), b.flow_per_step, (uint32_t)(b.M - 1), b.flow_gctr);
 }
 
 template <typename R>
@@ -2199,6 +2321,20 @@ static void batch_step(const BatchRun &b, hipStream_t s) {
 }
 
 void BatchRun::run(hipStream_t s, bool use_graph) {
+    if (flow) {
+        const void *fn = dtype == 0 ? reinterpret_cast<const void *>(&flow_batch_kernel<double>)
+                                    : reinterpret_cast<const void *>(&flow_batch_kernel<float>);
+        hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes), "flow LDS");
+        if (!use_graph) {
+            if (dtype == 0) batch_flow_launch<double>(*this, s);
+            else batch_flow_launch<float>(*this, s);
+            hip_check(hipGetLastError(), "batch dataflow launch");
+            return;
+        }
+        instantiate();
+        hip_check(hipGraphLaunch(flow_graph, s), "graph launch");
+        return;
+    }
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, jstep, done);
     if (dtype == 0)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<double>),
@@ -2238,6 +2374,25 @@ static hipGraphExec_t capture_steps(const BatchRun &b, uint64_t steps) {
 }
 
 void BatchRun::instantiate() {
+    if (flow) {
+        if (flow_graph) return;
+        const void *fn = dtype == 0 ? reinterpret_cast<const void *>(&flow_batch_kernel<double>)
+                                    : reinterpret_cast<const void *>(&flow_batch_kernel<float>);
+        hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes), "flow LDS");
+        hipStream_t cs;
+        hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+        hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (dtype == 0) batch_flow_launch<double>(*this, cs);
+        else batch_flow_launch<float>(*this, cs);
+        hipGraph_t g = nullptr;
+        hipError_t e1 = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        hip_check(e1, "end capture");
+        hipError_t e2 = hipGraphInstantiate(&flow_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e2, "graph instantiate");
+        return;
+    }
     if (graph || tail) return;
     if (dtype == 0)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_pass_kernel<double>),
@@ -2267,10 +2422,16 @@ void BatchRun::release() {
     if (tail) (void)hipGraphExecDestroy(tail);
     graph = tail = nullptr;
     graph_steps = graph_reps = tail_steps = 0;
-    void *ptrs[] = {args, work, fin, jstep, done};
+    if (flow_graph) (void)hipGraphExecDestroy(flow_graph);
+    flow_graph = nullptr;
+    void *ptrs[] = {args, work, fin, jstep, done, flow_args, flow_order, flow_gctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     args = nullptr; work = nullptr; fin = nullptr; jstep = nullptr; done = nullptr;
+    flow_args = nullptr; flow_order = nullptr; flow_gctr = nullptr;
+    flow = false;
+    flow_per_step = 0;
+    flow_plans.clear();
     launches.clear();
 }
 void evolve_in_place(const DevicePlan &dp, void *state, uint64_t block, uint64_t count, double *norms, double *scale,

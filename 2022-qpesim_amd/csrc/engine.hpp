@@ -94,8 +94,20 @@ struct BatchRun {
     static constexpr uint64_t kMaxGraphLaunches = 1u << 15;
     hipGraphExec_t graph = nullptr, tail = nullptr;
     uint64_t graph_steps = 0, graph_reps = 0, tail_steps = 0;
+    // Dataflow form (every member's plan allows it): the whole chain as one persistent
+    // launch whose claims interleave the members' items step by step (flow_order: one
+    // step's (member, item) claims); the per-member stamp resets and the launch are
+    // captured into flow_graph.
+    bool flow = false;
+    void *flow_args = nullptr;       // FlowArgs[count]
+    void *flow_order = nullptr;      // uint2[flow_per_step]
+    uint32_t flow_per_step = 0;
+    uint32_t *flow_gctr = nullptr;   // [4] launch counters
+    std::vector<const DevicePlan *> flow_plans;
+    hipGraphExec_t flow_graph = nullptr;
 
-    void build(const std::vector<BatchMember> &members, uint64_t M, hipStream_t s);
+    // use_flow: the dataflow form when every member's plan allows it
+    void build(const std::vector<BatchMember> &members, uint64_t M, hipStream_t s, bool use_flow = true);
     // captures the chain's graphs (once)
     void instantiate();
     // runs steps 1..M-1; with use_graph as graph launches

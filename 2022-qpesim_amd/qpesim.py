@@ -43,7 +43,7 @@ EXPORTS = [
     "qpe_time_evolve_iqpe", "qpe_state_apply_gate", "qpe_state_apply_controlled_gate", "qpe_qift_noalloc",
     "qpe_state_create_layout", "qpe_autotune_batch",
 ]
-BATCH_EAGER = 1
+BATCH_EAGER, BATCH_PASSES = 1, 2
 
 _lib = None
 
@@ -354,16 +354,19 @@ def time_evolve_iqpe(sv, h, num_of_applications):
     _call("qpe_time_evolve_iqpe", sv.handle, h.handle, int(num_of_applications))
 
 
-def batch_time_evolve(states, hams, eager=False):
+def batch_time_evolve(states, hams, eager=False, passes=False):
     """Batching form: evolve states[i] under hams[i] in one launch sequence (one
-    hipGraph launch unless eager).  Returns the device time in ms."""
+    hipGraph launch unless eager): one persistent dataflow launch when every
+    member's plan allows it, unless passes (then launch k runs pass k of every
+    member).  Returns the device time in ms."""
     n = len(states)
     if n != len(hams) or n == 0:
         raise ValueError("states and hams must be non-empty and of equal length")
     sa = (ctypes.c_void_p * n)(*[sv.handle.value for sv in states])
     ha = (ctypes.c_void_p * n)(*[h.handle.value for h in hams])
     ms = ctypes.c_double(0.0)
-    _call("qpe_batch_time_evolve", sa, ha, n, BATCH_EAGER if eager else 0, ctypes.byref(ms))
+    flags = (BATCH_EAGER if eager else 0) | (BATCH_PASSES if passes else 0)
+    _call("qpe_batch_time_evolve", sa, ha, n, flags, ctypes.byref(ms))
     return ms.value
 
 

@@ -211,12 +211,18 @@ int qpe_time_evolve_iqpe(qpe_state_t s, qpe_hamiltonian_t h, uint64_t num_of_app
  * Hamiltonian): time_evolve_over_terms for `count` independent instances in
  * one launch sequence.  states[i] is evolved under hams[i] exactly as
  * qpe_time_evolve_over_terms would (same results, bit for bit); all states
- * share the amplitude type and the ancilla count.  Launch k of a Trotter step
- * runs pass k of every instance; the whole chain of steps is captured into one
- * hipGraph and launched once (QPE_BATCH_EAGER: plain launches instead).
+ * share the amplitude type and the ancilla count.  When every instance's plan
+ * allows the dataflow evolution (QPE_OPT_EXECUTION), the whole chain of every
+ * instance is one persistent launch: its workgroups claim the instances' tile
+ * items step by step, interleaved, each starting as soon as the items it reads
+ * from are done, so instances of unequal size proceed at their own pace.
+ * Otherwise, or with QPE_BATCH_PASSES, launch k of a Trotter step runs pass k
+ * of every instance.  Either sequence is captured into a hipGraph and launched
+ * once (QPE_BATCH_EAGER: plain launches instead).
  * elapsed_ms (may be NULL): device time of the evolution.  Each state's
  * evolve timing is set to the batch time. */
 #define QPE_BATCH_EAGER 1
+#define QPE_BATCH_PASSES 2
 int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *hams, int count, int flags,
                           double *elapsed_ms);
 

@@ -140,3 +140,30 @@ def test_batch_autotuner_choices():
         assert np.array_equal(a, b)
     for sv in svs:
         sv.close()
+
+
+@pytest.mark.parametrize("dtype", [qpesim.COMPLEX128, qpesim.COMPLEX64])
+def test_batch_dataflow_equals_pass_chain(dtype):
+    """The persistent dataflow launch (default) and the chain of pass launches give
+    the same registers bit for bit; so do the dataflow launch's graph replays and an
+    eager launch, with eager work between the replays."""
+    m = 4
+    say(f"batch dataflow vs pass chain dtype={dtype}")
+    inst = _instances(m, dtype)
+    svs, hs = [i[0] for i in inst], [i[1] for i in inst]
+    qpesim.batch_time_evolve(svs, hs, passes=True)
+    ref = _all(inst)
+    assert all(sv.timings()["launches_per_step"] > 1 for sv in svs)
+    _reset(inst, m)
+    qpesim.batch_time_evolve(svs, hs, eager=True)
+    assert all(sv.timings()["launches_per_step"] == 1 for sv in svs)
+    for a, b in zip(ref, _all(inst)):
+        assert np.array_equal(a, b)
+    for rep in range(3):
+        _reset(inst, m)
+        _ = [sv.block_norms() for sv in svs]
+        qpesim.batch_time_evolve(svs, hs)
+        for a, b in zip(ref, _all(inst)):
+            assert np.array_equal(a, b), f"replay {rep}"
+    for sv in svs:
+        sv.close()
