@@ -184,8 +184,9 @@ def measured_traffic(workload_key):
 
 def bench_batch(a):
     """Batching form: 48 Hamiltonians of uneven size (n in {12..24}, N in {4,6,8},
-    m=8) evolved in FP32 by one graph launch; per-instance max|dP| vs FP64.
-    A step is one whole batched evolution (all 255 Trotter steps of every instance)."""
+    m=8) evolved in FP32 by one persistent dataflow launch (captured in a hipGraph)
+    after qpe_autotune_batch; per-instance max|dP| vs FP64.  A step is one whole
+    batched evolution (all 255 Trotter steps of every instance)."""
     specs = qpesim.batch_instances()
     m = a.m if a.m is not None else specs[0]["m"]
     M = 1 << m
@@ -238,8 +239,8 @@ def bench_batch(a):
         "vs_baseline": None, "dtype": "complex64 (fp32 state/rotations; fp64 angles, norms, P(k))",
         "data": "synthetic: 48 seeded Hamiltonians (SplitMix64/Box-Muller), Hartree-Fock initial states",
         "config": {"workload": f"batching form: 48 QPE instances, n=12..24 spin-orbitals, N=4..8, m={m}, "
-                               "one hipGraph launch per batch evolution", "instances": len(specs),
-                   "ancillas": m, "parallelism": "single GPU"},
+                               "one persistent dataflow launch (hipGraph-captured) per batch evolution",
+                   "instances": len(specs), "ancillas": m, "parallelism": "single GPU"},
         "batch_evolve_ms": ev_ms, "qift_pk_s": post_s, "updates_per_batch": tot_upd,
     }
     if not a.no_fp64:
