@@ -611,7 +611,7 @@ static void prepare_evolve(qpe_state_t s, qpe_hamiltonian_t h, int term_order) {
             "hamiltonian (num_orbitals, num_occupied) does not match the state");
     materialize_scale(s);
     if (!s->plan || s->plan_h != h || s->plan_version != h->version || s->plan_term_order != term_order) {
-        size_t budget = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
+        size_t budget = (size_t)s->max_lds > qpe::kLdsHeader ? (size_t)s->max_lds - qpe::kLdsHeader : 0;
         if (s->max_tile_bytes > 0) budget = std::min(budget, (size_t)s->max_tile_bytes);
         qpe::Plan p;
         try {
@@ -829,7 +829,7 @@ std::vector<TuneCandidate> tune_candidates(qpe_state_t s, qpe_hamiltonian_t h, i
     check_ham(h);
     require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
     require(s->layout == QPE_LAYOUT_ALPHA_BETA, QPE_ERR_ARG, "the autotuner needs the alpha x beta layout");
-    const size_t full = (size_t)s->max_lds > 4096 ? (size_t)s->max_lds - 4096 : 0;
+    const size_t full = (size_t)s->max_lds > qpe::kLdsHeader ? (size_t)s->max_lds - qpe::kLdsHeader : 0;
     const size_t line = (size_t)s->C * s->elem;
     std::vector<size_t> budgets;
     for (size_t w = std::min<size_t>(full / line, 64); w >= 1; w /= 2) budgets.push_back(std::min(full, w * line + 64));
@@ -1134,7 +1134,7 @@ int qpe_plan_info(qpe_hamiltonian_t h, int dtype, uint64_t tile_bytes, double ou
         require(out != nullptr, QPE_ERR_ARG, "null output");
         require(dtype == QPE_COMPLEX128 || dtype == QPE_COMPLEX64, QPE_ERR_ARG, "unknown dtype");
         require(h->h.indexed(), QPE_ERR_STATE, "IndexTerms has not been called");
-        size_t budget = tile_bytes ? (size_t)tile_bytes : (size_t)(160 * 1024 - 4096);
+        size_t budget = tile_bytes ? (size_t)tile_bytes : (size_t)(160 * 1024 - qpe::kLdsHeader);
         qpe::Plan p;
         try {
             p = qpe::build_plan(h->h, dtype == QPE_COMPLEX128 ? 16 : 8, budget);

@@ -132,6 +132,7 @@ struct PassArgs {
     const ItemDev *items;
     const uint32_t *col_list;
     const uint32_t *sched;
+    const uint32_t *pass_qr;   // q | r << 8 of the mixed windows' pair lists
     const double *coef;   // [op][kMaxGroup] (1 - cos, sin), op-major, in the state's type (float pairs for complex64)
     double *partials;
     const double *scale;
@@ -259,12 +260,12 @@ __device__ inline void load_entries(const PassArgs &A, const OpDev &d, bool vali
 // partner l10, as byte offsets line_base[l] * line_bytes from the tile base
 // (line_base: the line's position in LDS, or its column rank when the tile is
 // the block in global memory); bit 0 of .x holds the Jordan-Wigner parity of
-// the pair (offsets are multiples of 8).
+// the pair (offsets are multiples of 8).  Run by one whole wave.
 __device__ inline void build_pair_list(uint32_t qr, const uint32_t *line_str, const uint32_t *line_base,
                                        int2 *pair_list, int *pair_count, int nl, uint32_t line_bytes) {
-    const int tid = threadIdx.x;
-    const int q = (int)(qr & 0xff), r = (int)(qr >> 8);
-    if (tid < 64) {
+    const int tid = threadIdx.x & 63;
+    const int q = (int)(qr & 0xff), r = (int)((qr >> 8) & 0xff);
+    {
         // lane l holds line l's string (no string has all 32 bits set); the partner
         // search reads the other lanes' strings with readlane, not LDS round trips
         const uint32_t s = tid < nl ? line_str[tid] : ~0u;
@@ -424,8 +425,9 @@ __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int la
 template <typename R>
 __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename Vec<R>::type *coef, const EntSet &E,
                               typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
-                              const uint32_t *line_str, const uint32_t *line_base, int2 *pair_list, int *pair_count,
-                              int nl, uint32_t C, uint32_t &cur_qr, int &ncp, uint32_t tseed) {
+                              const uint32_t *line_str, const uint32_t *line_base, const int2 *pair_lists,
+                              const int *pair_counts, int nl, uint32_t C, uint32_t &cur_list, int &ncp,
+                              uint32_t tseed) {
     using V = typename Vec<R>::type;
     const int tid = threadIdx.x;
     const int type = d.type;
@@ -437,12 +439,13 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
     const Dither z = make_dither(tseed);
     const uint32_t lbase = __umul24((uint32_t)E.lane, line_bytes);   // the thread's first line (line-local ops)
     if (type == OP_FAB) {
-        if (d.aux != cur_qr) {
-            cur_qr = d.aux;
-            build_pair_list(cur_qr, line_str, line_base, pair_list, pair_count, nl, line_bytes);
-            lds_barrier();
-            ncp = *pair_count;
+        // the pass's pair lists were built when the item started; aux bits 16-18 pick this op's
+        const uint32_t li = (d.aux >> 16) & (uint32_t)(kMaxPairLists - 1);
+        if (li != cur_list) {
+            cur_list = li;
+            ncp = pair_counts[li];
         }
+        const int2 *pair_list = pair_lists + li * (kMaxLines / 2);
         if (E.lane >= E.L) return;
         if (ncp <= kChunk * E.L) {
             // the thread's column pairs fit one chunk: read them once for all entries
@@ -714,13 +717,13 @@ template <typename R, bool GLOBAL>
 __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, uint32_t item, uint64_t j,
                                         typename Vec<R>::type *gtile, OpDev *ring, double2 *cring, EntSet &E0,
                                         EntSet &E1, const double2 *gang, const uint32_t *line_rank,
-                                        const uint32_t *line_str, const uint32_t *line_base, int2 *pair_list,
-                                        int *pair_count, int nl, uint32_t C) {
+                                        const uint32_t *line_str, const uint32_t *line_base, const int2 *pair_lists,
+                                        const int *pair_counts, int nl, uint32_t C) {
     using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    V *tile = GLOBAL ? gtile : reinterpret_cast<V *>(smem + 4096);
+    V *tile = GLOBAL ? gtile : reinterpret_cast<V *>(smem + kLdsHeader);
     const int o0 = pass.op_begin, o1 = pass.op_end;
-    uint32_t cur_qr = 0xffffffffu;
+    uint32_t cur_list = 0xffffffffu;   // pair list of the last mixed-window op, and its count
     int ncp = 0;
     const int nbatch = (o1 - o0 + kBatch - 1) / kBatch;
     for (int b = 0; b < nbatch; ++b) {
@@ -743,8 +746,8 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
                 // at least one barrier earlier)
                 const OpDev d = rb[i];
                 op_barrier(GLOBAL);
-                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_list,
-                          pair_count, nl, C, cur_qr, ncp,
+                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_lists,
+                          pair_counts, nl, C, cur_list, ncp,
                           mix32(mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u ^ item * 0x9E3779B1u) ^ threadIdx.x));
             }
             if (o + 1 >= o1) break;
@@ -754,8 +757,8 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
             {
                 const OpDev d = rb[i + 1];
                 op_barrier(GLOBAL);
-                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_list,
-                          pair_count, nl, C, cur_qr, ncp,
+                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_lists,
+                          pair_counts, nl, C, cur_list, ncp,
                           mix32(mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u ^ item * 0x9E3779B1u) ^
                               threadIdx.x));
             }
@@ -779,21 +782,24 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     static_assert(!FLOW || MODE == 1, "dataflow items are LDS-tiled");
     using V = typename Vec<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS header (bytes): line ranks [0,256), line strings [256,512), column pair list [512,1024)
-    // (reused for the norm reduction after the ops), pair count, LDS line index [1040,1296),
-    // descriptor ring [1296,1808), coefficient ring [1808,3856); the tile starts at 4096
+    // LDS header (bytes): line ranks [0,256), line strings [256,512), norm reduction
+    // [512,768), LDS line index [1040,1296), descriptor ring [1296,1808), coefficient ring
+    // [1808,3856), dataflow broadcast words [3856,3868), the mixed windows' pair lists
+    // [4096,6144) and their counts [6144,6176); the tile starts at kLdsHeader
     uint32_t *line_rank = reinterpret_cast<uint32_t *>(smem);            // [kMaxLines]
     uint32_t *line_str = line_rank + kMaxLines;                         // [kMaxLines]
-    int2 *pair_list = reinterpret_cast<int2 *>(line_str + kMaxLines);   // [kMaxLines]
-    double *red = reinterpret_cast<double *>(pair_list);                // [2 * kWaves], after the ops
-    int *pair_count = reinterpret_cast<int *>(smem + 1024);             // [4]
+    double *red = reinterpret_cast<double *>(smem + 512);               // [2 * kWaves], after the ops
+    int2 *pair_lists = reinterpret_cast<int2 *>(smem + 4096);           // [kMaxPairLists][kMaxLines / 2]
+    int *pair_counts = reinterpret_cast<int *>(smem + 6144);            // [kMaxPairLists]
+    static_assert(4096 + kMaxPairLists * (kMaxLines / 2) * 8 == 6144 && 6144 + 4 * kMaxPairLists <= (int)kLdsHeader,
+                  "pair lists overlap the tile");
     OpDev *ring = reinterpret_cast<OpDev *>(smem + 1296);              // [2][kBatch]
     double2 *cring = reinterpret_cast<double2 *>(smem + 1808);          // [2][kBatch][kMaxGroup]
-    static_assert(16 * kWaves <= 8 * kMaxLines, "norm reduction does not fit the pair list");
-    static_assert(1808 + 2 * kBatch * kMaxGroup * 16 <= 4096, "LDS header overlaps the tile");
+    static_assert(16 * kWaves <= 256, "norm reduction does not fit its LDS region");
+    static_assert(1808 + 2 * kBatch * kMaxGroup * 16 + 16 <= 4096, "LDS header overlaps the pair lists");
     // lines stay in global memory (MODE 1: never, 2: always, 0: as the pass says)
     const bool gmode = MODE == 2 ? true : MODE == 1 ? false : pass.kind == PASS_COLG;
-    V *tile = reinterpret_cast<V *>(smem + 4096);
+    V *tile = reinterpret_cast<V *>(smem + kLdsHeader);
     uint32_t *line_pos = reinterpret_cast<uint32_t *>(smem + 1040);     // [kMaxLines] LDS line index (0..63)
 
     const int tid = threadIdx.x;
@@ -801,7 +807,7 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     const int nl = (int)it.count;
     const uint32_t C = A.C;
     const bool col = pass.kind != PASS_ROW;   // COL and COLG lines are columns from the column list
-    double2 *gang = reinterpret_cast<double2 *>(smem + 4096 + (size_t)pass.max_lines * C * sizeof(V));
+    double2 *gang = reinterpret_cast<double2 *>(smem + kLdsHeader + (size_t)pass.max_lines * C * sizeof(V));
     const int o0 = pass.op_begin, o1 = pass.op_end;
 
     if (tid < nl) {
@@ -812,6 +818,15 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     }
     commit_batch<R>(fetch_batch<R>(A, o0, o1, 0), ring, cring, 0);
     __syncthreads();
+    // mixed windows: wave w builds the pair list of the pass's w-th (q, r); the first op's
+    // barrier publishes them
+    {
+        const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+        if (w < pass.pair_lists)
+            build_pair_list(A.pass_qr[pass.pair_qr + w], line_str, gmode ? line_rank : line_pos,
+                            pair_lists + w * (kMaxLines / 2), pair_counts + w, nl,
+                            C * (uint32_t)sizeof(V));
+    }
     for (int i = tid; i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
         const double *gb = A.grp_b + 2ull * (pass.group_first + g) * C + 2ull * line_rank[l];
@@ -841,11 +856,11 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     // two instantiations, so that each sees a single address space for the tile
     // (LDS accesses stay ds_read/ds_write, global ones global_load/store)
     if (gmode)
-        op_loop<R, true>(A, pass, item, j, cur, ring, cring, E0, E1, gang, line_rank, line_str, line_rank, pair_list,
-                         pair_count, nl, C);
+        op_loop<R, true>(A, pass, item, j, cur, ring, cring, E0, E1, gang, line_rank, line_str, line_rank, pair_lists,
+                         pair_counts, nl, C);
     else
         op_loop<R, false>(A, pass, item, j, nullptr, ring, cring, E0, E1, gang, line_rank, line_str, line_pos,
-                          pair_list, pair_count, nl, C);
+                          pair_lists, pair_counts, nl, C);
     __syncthreads();
 
     // ---- FP64 norm of the tile (last pass of a step) ----------------------
@@ -915,6 +930,7 @@ struct PassArgsG {
     QPE_GPTR(const ItemDev) items;
     QPE_GPTR(const uint32_t) col_list;
     QPE_GPTR(const uint32_t) sched;
+    QPE_GPTR(const uint32_t) pass_qr;
     QPE_GPTR(const double) coef;
     QPE_GPTR(double) partials;
     QPE_GPTR(const double) scale;
@@ -932,7 +948,8 @@ __device__ inline PassArgs from_global_args(const PassArgsG &g) {
     A.theta_a = (const double *)g.theta_a; A.theta_b = (const double *)g.theta_b; A.cross = (const double *)g.cross;
     A.groups = (const GroupDev *)g.groups; A.grp_a = (const double *)g.grp_a; A.grp_b = (const double *)g.grp_b;
     A.ops = (const OpDev *)g.ops; A.items = (const ItemDev *)g.items; A.col_list = (const uint32_t *)g.col_list;
-    A.sched = (const uint32_t *)g.sched; A.coef = (const double *)g.coef; A.partials = (double *)g.partials;
+    A.sched = (const uint32_t *)g.sched; A.pass_qr = (const uint32_t *)g.pass_qr;
+    A.coef = (const double *)g.coef; A.partials = (double *)g.partials;
     A.scale = (const double *)g.scale; A.state = (void *)g.state;
     A.D = g.D; A.n = g.n; A.e = g.e; A.C = g.C; A.ca_pairs = g.ca_pairs; A.ccaa_pairs = g.ccaa_pairs;
     A.renorm = g.renorm;
@@ -1960,6 +1977,7 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     to_device(&items, p.items, s);
     to_device(&col_list, p.col_list, s);
     to_device(&sched, p.sched, s);
+    to_device(&pass_qr, p.pass_qr, s);
     // coefficients op-major, kMaxGroup slots per op (slot k of an op: term index + k),
     // so that staging an op's coefficients needs no descriptor first; stored in the
     // state's type (complex64: rounded to float here, once)
@@ -1987,7 +2005,7 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     for (const auto &ps : p.passes) {
         if (ps.max_lines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
         const size_t tile = ps.kind == PASS_COLG ? 0 : (size_t)ps.max_lines * p.C * p.elem_bytes;
-        size_t b = 4096 + tile + (size_t)ps.group_count * ps.max_lines * 16;
+        size_t b = kLdsHeader + tile + (size_t)ps.group_count * ps.max_lines * 16;
         pass_lds.push_back(b);
         lds_bytes = std::max(lds_bytes, b);
     }
@@ -2014,8 +2032,8 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
 
 void DevicePlan::release() {
     void *ptrs[] = {binom, strings, theta_a, theta_b, cross, groups, grp_a, grp_b, ops, items, col_list,
-                    partials, sched, coef, flow_item_pass, flow_pass_first, flow_dep_off, flow_deps, flow_passes,
-                    flow_done, flow_ctr};
+                    partials, sched, pass_qr, coef, flow_item_pass, flow_pass_first, flow_dep_off, flow_deps,
+                    flow_passes, flow_done, flow_ctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     flow_ok = false;
@@ -2026,6 +2044,7 @@ void DevicePlan::release() {
     binom = nullptr; strings = nullptr; theta_a = theta_b = cross = nullptr; groups = nullptr;
     grp_a = grp_b = nullptr; ops = nullptr; items = nullptr; col_list = nullptr; partials = nullptr;
     sched = nullptr;
+    pass_qr = nullptr;
     coef = nullptr;
 }
 
@@ -2036,7 +2055,7 @@ static PassArgs make_args(const DevicePlan &dp, void *state, double *scale, int 
     A.theta_a = dp.theta_a; A.theta_b = dp.theta_b; A.cross = dp.cross;
     A.groups = dp.groups; A.grp_a = dp.grp_a; A.grp_b = dp.grp_b;
     A.ops = dp.ops; A.items = dp.items; A.col_list = dp.col_list;
-    A.sched = dp.sched; A.coef = dp.coef;
+    A.sched = dp.sched; A.pass_qr = dp.pass_qr; A.coef = dp.coef;
     A.partials = dp.partials; A.scale = scale;
     A.state = state;
     A.D = (uint64_t)p.C * p.C;

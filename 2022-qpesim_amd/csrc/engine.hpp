@@ -26,6 +26,7 @@ struct DevicePlan {
     ItemDev *items = nullptr;
     uint32_t *col_list = nullptr;
     uint32_t *sched = nullptr;
+    uint32_t *pass_qr = nullptr;   // mixed windows' pair lists (Plan::pass_qr)
     double *coef = nullptr;      // [op][kMaxGroup][2]: (1 - cos, sin) of each op's terms, in the
                                  // state's type (float pairs for complex64)
     double *partials = nullptr;   // [max_items] norm partial sums

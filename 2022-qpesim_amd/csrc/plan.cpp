@@ -646,7 +646,27 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         pd.max_lines = max_lines;
         pd.group_first = gfirst;
         pd.group_count = gcount;
-        pd.pad0 = pd.pad1 = pd.pad2 = 0;
+        pd.pair_lists = 0;
+        pd.pair_qr = 0;
+        pd.pad2 = 0;
+        // mixed windows: the distinct (q, r) of the pass's ops, in order, each op's index in aux
+        {
+            std::vector<uint32_t> qr;
+            for (size_t o = (size_t)op_begin; o < P.ops.size(); ++o) {
+                OpDev &op = P.ops[o];
+                if (op.type != OP_FAB) continue;
+                size_t k = 0;
+                while (k < qr.size() && qr[k] != op.aux) ++k;
+                if (k == qr.size()) qr.push_back(op.aux);
+                if (k >= (size_t)kMaxPairLists) throw std::runtime_error("a mixed window with too many (q, r)");
+                op.aux |= (uint32_t)k << 16;
+            }
+            if (!qr.empty()) {
+                pd.pair_lists = (int32_t)qr.size();
+                pd.pair_qr = (int32_t)P.pass_qr.size();
+                P.pass_qr.insert(P.pass_qr.end(), qr.begin(), qr.end());
+            }
+        }
         for (const auto &it : items) {
             P.items.push_back(it);
             if (kind != PASS_COLG) P.tile_bytes_moved += 2ull * it.count * line_bytes;
@@ -745,6 +765,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
                 int r = abba[aj].r;
                 uint32_t nm = tmask | (1u << r);
                 if (nm != tmask) {
+                    if (popcnt(nm) > kMaxPairLists + 1) break;   // q and at most kMaxPairLists r
                     int ts = popcnt(nm);
                     int k0 = std::max(1, e - (n - ts)), k1 = std::min(e, ts - 1);
                     if (max_orbit(ts, k0, k1) > (uint64_t)WW) break;
@@ -829,6 +850,7 @@ Plan build_plan(const Hamiltonian &h, size_t elem_bytes, size_t tile_bytes, bool
         P.diag_updates *= 2;
     }
     if (P.sched.empty()) P.sched.push_back(pack_sched(kSchedPad, kSchedPad, 0, 0));
+    if (P.pass_qr.empty()) P.pass_qr.push_back(0u);
     // the block norm is summed by the last pass; it must see every amplitude
     {
         const PassDev &last = P.passes.back();

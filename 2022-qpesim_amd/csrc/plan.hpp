@@ -49,6 +49,12 @@ enum OpType : int32_t {
 // excitations when the LDS tile cannot hold two columns); its ops are FAB only.
 enum PassKind : int32_t { PASS_COL = 0, PASS_ROW = 1, PASS_COLG = 2 };
 constexpr int kMaxLinesPlan = 64;   // lines per workgroup (line tables in LDS)
+// Column-pair lists of a mixed-window pass: one per distinct (q, r) of its ops, at most
+// kMaxPairLists (a window holds at most that many r values), each of at most
+// kMaxLinesPlan / 2 pairs; built in LDS when an item starts.
+constexpr int kMaxPairLists = 8;
+// LDS bytes of the pass kernel's header (line tables, pair lists, op rings) before the tile
+constexpr size_t kLdsHeader = 6208;
 
 enum PassFlags : int32_t {
     PF_FROM_PREV = 1,  // read block j-1 (scaled by its renormalisation factor), write block j
@@ -66,7 +72,8 @@ struct OpDev {
     int32_t index;   // FCC/FAB/PQQR: first term (coefficients coef[index .. index+g)); GA/GB: offset of
                      // the pair angles; DIAG: diagonal table set
     uint32_t tab;    // offset of the schedule
-    uint32_t aux;    // GA/GB: staged-angle slot; FAB: q | r << 8 (beta orbitals); PQQR: line gate mask
+    uint32_t aux;    // GA/GB: staged-angle slot; FAB: q | r << 8 (beta orbitals) | pair list << 16;
+                     // PQQR: line gate mask
     int32_t lg;      // schedule rows
     int32_t ns;      // schedule slots (kThreadsPlan)
     int32_t g;       // terms in the group
@@ -129,7 +136,9 @@ struct PassDev {
     int32_t max_lines;    // columns (COL) or rows (ROW) per workgroup
     int32_t group_first;  // first excitation group whose angle table is staged in LDS
     int32_t group_count;  // number of staged groups (0: none)
-    int32_t pad0, pad1, pad2;
+    int32_t pair_lists;   // mixed windows: distinct (q, r) of the ops (FAB op aux bits 16-18 index them)
+    int32_t pair_qr;      // offset of their q | r << 8 values in Plan::pass_qr
+    int32_t pad2;
 };
 
 struct ItemDev {
@@ -159,8 +168,9 @@ struct Plan {
     std::vector<uint32_t> col_list;     // column ranks of COL items
     std::vector<uint32_t> ca_table;     // host only: [(x*n+y)*ca_pairs + k]: rank(f|x), rank(f|y), parity
     std::vector<uint32_t> sched;        // op schedules: [op.tab + i*op.ns + slot]
+    std::vector<uint32_t> pass_qr;      // mixed windows' pair lists: q | r << 8, per pass from pair_qr
     std::vector<double> coef;           // [term][2]: 1-cos(c t), sin(c t) of every double excitation
-    size_t lds_fixed = 4096;            // LDS bytes before the tile
+    size_t lds_fixed = kLdsHeader;      // LDS bytes before the tile
 
     // work accounting per U
     uint64_t pair_updates = 0;   // determinant-pair rotations of the reference's terms

@@ -18,7 +18,7 @@ import qpesim
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "..", "2022-qpesim_amd", "csrc")
-FULL = 160 * 1024 - 4096   # the pass kernel's LDS tile budget
+FULL = 160 * 1024 - 6208   # the pass kernel's LDS tile budget (160 KB less its header)
 
 
 @pytest.fixture(scope="module")
