@@ -810,6 +810,8 @@ int qpe_batch_time_evolve(const qpe_state_t *states, const qpe_hamiltonian_t *ha
 
 // ---- autotuner ------------------------------------------------------------------
 
+}  // extern "C"
+
 namespace {
 // one timed configuration of an instance
 struct TuneCandidate {
@@ -884,6 +886,8 @@ void apply_choice(qpe_state_t s, const TuneCandidate &c, double *result) {
 }
 }  // namespace
 
+extern "C" {
+
 int qpe_autotune(qpe_state_t s, qpe_hamiltonian_t h, int share_cus, double *result) {
     return guarded([&] {
         require(share_cus >= 1, QPE_ERR_ARG, "share_cus must be positive");
@@ -922,7 +926,7 @@ int qpe_autotune_batch(const qpe_state_t *states, const qpe_hamiltonian_t *hams,
                     if (cands[i][k].passes < cands[i][best].passes) best = k;
             return best;
         };
-        // Three policies, each timed as one batched step on two-block scratch states:
+        // Four policies, each timed as one batched step on two-block scratch states:
         //  A  every instance at its fewest passes (the fastest of those): the fewest launches;
         //  B  the instance whose fastest step is slowest at that fastest candidate, every other
         //     instance at its fastest candidate with no more passes than that one;

@@ -2465,8 +2465,9 @@ void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end,
         throw std::runtime_error("the dataflow evolution needs LDS-tiled passes and blocks below 2^31 bytes");
     // auto: the dataflow kernel for complex128, whose tiles are half as wide (twice the
     // items per pass, so more passes overlap); complex64 measured faster pass by pass
-    // (32 spin-orbitals / 8 electrons: 60.0 against 70.7 ms per complex128 step, 38.3
-    // against 37.1 ms per complex64 step)
+    // (32 spin-orbitals / 8 electrons: 58.9 against 68.2 ms per complex128 step, 36.7
+    // against 35.0 ms per complex64 step; 28 spin-orbitals / 10 electrons: 46.3 against
+    // 48.6 and 30.3 against 26.5 ms)
     const bool flow = execution == EXEC_DATAFLOW || (execution == EXEC_AUTO && dp.dtype == 0);
     if (flow && dp.flow_ok) {
         if (dp.dtype == 0)
