@@ -166,6 +166,7 @@ __device__ inline void op_barrier(bool global_tile) {
 // static count so the compiler's wait counts stay exact) are loaded while the
 // previous op runs, into two alternating register sets.
 constexpr int kBatch = 8;
+constexpr int kOpsAll = 0, kOpsFab = 1, kOpsFcc = 2;   // op-type sets of a pass kernel instantiation
 constexpr int kDescWords = sizeof(OpDev) / 4;   // 8
 // lines (column pairs) whose LDS loads are issued together: 2 rather than 4 keeps the kernel at
 // 112 VGPRs and 81 KB of code (measured 11% faster per Trotter step; 1 gains nothing more)
@@ -422,7 +423,11 @@ __device__ inline void rotate_group(V *tile, uint32_t line_bytes, int nl, int la
     }
 }
 
-template <typename R>
+// OPS: the op types a kernel carries: kOpsAll, kOpsFab (passes of mixed windows, whose
+// ops are all FAB) or kOpsFcc (same-spin passes, all FCC).  A kernel with one op path has
+// 40% less code and 20 fewer VGPRs, which measured 3.4% off a complex64 step at 32
+// spin-orbitals / 8 electrons for the FAB passes alone.
+template <typename R, int OPS = kOpsAll>
 __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename Vec<R>::type *coef, const EntSet &E,
                               typename Vec<R>::type *tile, const double2 *gang, const uint32_t *line_rank,
                               const uint32_t *line_str, const uint32_t *line_base, const int2 *pair_lists,
@@ -438,6 +443,19 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
     // 3.6e-6 against 7.9e-7 at 32 spin-orbitals / 8 electrons / 10 ancillas
     const Dither z = make_dither(tseed);
     const uint32_t lbase = __umul24((uint32_t)E.lane, line_bytes);   // the thread's first line (line-local ops)
+    if constexpr (OPS == kOpsFcc) {
+        if (E.lane >= E.L) return;
+#pragma unroll
+        for (int i = 0; i < kSchedMax; ++i) {
+            if (i >= d.lg) break;
+            const uint32_t e = E.e[i];
+            if (s_rank1(e) == kSchedPad) continue;
+            const V cf = coef[s_slot(e)];
+            rotate_lines<R>(tile, line_bytes, nl, E.lane, E.L, lbase, s_rank1(e), s_rank2(e), cf.x,
+                            flip_sign(cf.y, ~s_par(e) << 31), entry_dither(z, i));
+        }
+        return;
+    }
     if (type == OP_FAB) {
         // the pass's pair lists were built when the item started; aux bits 16-18 pick this op's
         const uint32_t li = (d.aux >> 16) & (uint32_t)(kMaxPairLists - 1);
@@ -480,7 +498,8 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
             rotate_pairs<R>(tile, pair_list, ncp, E.lane, E.L, s_rank1(e), s_rank2(e), s_par(e), cf.x, cf.y,
                             entry_dither(z, i));
         }
-    } else if (type == OP_FCC) {
+    } else if constexpr (OPS == kOpsAll) {
+      if (type == OP_FCC) {
         if (E.lane >= E.L) return;
 #pragma unroll
         for (int i = 0; i < kSchedMax; ++i) {
@@ -538,6 +557,7 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
             w.y = upd<R>(v.y, -s * v.x - h * v.y, dfac(r, 1));
             tile[idx] = w;
         }
+      }
     }
 }
 
@@ -713,7 +733,7 @@ __device__ __forceinline__ void copy_cols_out(V *dst, const V *tile, const uint3
 // descriptors and coefficients were staged in the LDS rings; each op's
 // schedule entries load one op ahead into the other register set.  GLOBAL: the
 // lines are columns of the block in global memory (gtile), else the LDS tile.
-template <typename R, bool GLOBAL>
+template <typename R, bool GLOBAL, int OPS = kOpsAll>
 __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, uint32_t item, uint64_t j,
                                         typename Vec<R>::type *gtile, OpDev *ring, double2 *cring, EntSet &E0,
                                         EntSet &E1, const double2 *gang, const uint32_t *line_rank,
@@ -746,7 +766,7 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
                 // at least one barrier earlier)
                 const OpDev d = rb[i];
                 op_barrier(GLOBAL);
-                run_op<R>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_lists,
+                run_op<R, OPS>(A, d, cb + i * kMaxGroup, E0, tile, gang, line_rank, line_str, line_base, pair_lists,
                           pair_counts, nl, C, cur_list, ncp,
                           mix32(mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)o * 0x165667B1u ^ item * 0x9E3779B1u) ^ threadIdx.x));
             }
@@ -757,7 +777,7 @@ __device__ __forceinline__ void op_loop(const PassArgs &A, const PassDev &pass, 
             {
                 const OpDev d = rb[i + 1];
                 op_barrier(GLOBAL);
-                run_op<R>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_lists,
+                run_op<R, OPS>(A, d, cb + (i + 1) * kMaxGroup, E1, tile, gang, line_rank, line_str, line_base, pair_lists,
                           pair_counts, nl, C, cur_list, ncp,
                           mix32(mix32((uint32_t)j * 0x27D4EB2Fu ^ (uint32_t)(o + 1) * 0x165667B1u ^ item * 0x9E3779B1u) ^
                               threadIdx.x));
@@ -775,7 +795,7 @@ struct NoGate {
 // factor and the norm partial coherent at device scope; LDS-tiled passes only).
 // gate() runs, in every thread, after the item's tables and first op are staged
 // and before its tile is read; false abandons the item (returns false).
-template <typename R, int MODE, bool FLOW = false, class Gate = NoGate>
+template <typename R, int MODE, bool FLOW = false, class Gate = NoGate, int OPS = kOpsAll>
 __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass, uint32_t item,
                                           typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j,
                                           Gate gate = Gate()) {
@@ -856,10 +876,10 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     // two instantiations, so that each sees a single address space for the tile
     // (LDS accesses stay ds_read/ds_write, global ones global_load/store)
     if (gmode)
-        op_loop<R, true>(A, pass, item, j, cur, ring, cring, E0, E1, gang, line_rank, line_str, line_rank, pair_lists,
+        op_loop<R, true, OPS>(A, pass, item, j, cur, ring, cring, E0, E1, gang, line_rank, line_str, line_rank, pair_lists,
                          pair_counts, nl, C);
     else
-        op_loop<R, false>(A, pass, item, j, nullptr, ring, cring, E0, E1, gang, line_rank, line_str, line_pos,
+        op_loop<R, false, OPS>(A, pass, item, j, nullptr, ring, cring, E0, E1, gang, line_rank, line_str, line_pos,
                           pair_lists, pair_counts, nl, C);
     __syncthreads();
 
@@ -905,10 +925,10 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
 
 // MODE 1: LDS-tiled passes only, MODE 2: global-memory windows only (separate
 // kernels keep the common one small)
-template <typename R, int MODE>
+template <typename R, int MODE, int OPS = kOpsAll>
 __global__ void __launch_bounds__(kThreads)
 pass_kernel(PassArgs A, PassDev pass, typename Vec<R>::type *prev, typename Vec<R>::type *cur, uint64_t j) {
-    pass_body<R, MODE>(A, pass, blockIdx.x, prev, cur, j);
+    pass_body<R, MODE, false, NoGate, OPS>(A, pass, blockIdx.x, prev, cur, j);
 }
 
 // Pointers loaded from a table carry no address space, so the compiler would
@@ -1156,9 +1176,15 @@ __device__ __forceinline__ bool flow_item(const PassArgs &A, const FlowArgs &F, 
         __syncthreads();
         return bc[1] != 0u;
     };
-    if (!pass_body<R, 1, true>(A, ps, x - F.pass_first[k], base + (uint64_t)(j - 1) * A.D, base + (uint64_t)j * A.D,
-                               j, gate))
-        return false;
+    // the item runs the body compiled for its pass's op types (pad2): one op path in the
+    // hot loop, as in the pass launches
+    const uint32_t item = x - F.pass_first[k];
+    V *prev = base + (uint64_t)(j - 1) * A.D, *cur = base + (uint64_t)j * A.D;
+    bool ok;
+    if (ps.pad2 == kOpsFab) ok = pass_body<R, 1, true, decltype(gate), kOpsFab>(A, ps, item, prev, cur, j, gate);
+    else if (ps.pad2 == kOpsFcc) ok = pass_body<R, 1, true, decltype(gate), kOpsFcc>(A, ps, item, prev, cur, j, gate);
+    else ok = pass_body<R, 1, true, decltype(gate)>(A, ps, item, prev, cur, j, gate);
+    if (!ok) return false;
     // every thread's block stores and the norm partial have completed before the stamp
     __builtin_amdgcn_s_waitcnt(kVmcntZero);
     __syncthreads();
@@ -2002,7 +2028,14 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     hip_check(hipMalloc(reinterpret_cast<void **>(&partials), std::max(1, p.max_items) * sizeof(double)), "hipMalloc(partials)");
     lds_bytes = 0;
     pass_lds.clear();
+    pass_ops.clear();
     for (const auto &ps : p.passes) {
+        bool fab = ps.kind == PASS_COL && ps.op_end > ps.op_begin, fcc = ps.kind != PASS_COLG && ps.op_end > ps.op_begin;
+        for (int o = ps.op_begin; o < ps.op_end; ++o) {
+            fab = fab && p.ops[o].type == OP_FAB;
+            fcc = fcc && p.ops[o].type == OP_FCC;
+        }
+        pass_ops.push_back(fab ? kOpsFab : fcc ? kOpsFcc : kOpsAll);
         if (ps.max_lines > kMaxLines) throw std::runtime_error("plan exceeds the per-tile line limit");
         const size_t tile = ps.kind == PASS_COLG ? 0 : (size_t)ps.max_lines * p.C * p.elem_bytes;
         size_t b = kLdsHeader + tile + (size_t)ps.group_count * ps.max_lines * 16;
@@ -2022,7 +2055,10 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
         std::vector<uint32_t> deps = fl.deps;
         if (deps.empty()) deps.push_back(0u);
         to_device(&flow_deps, deps, s);
-        to_device(&flow_passes, p.passes, s);
+        // the dataflow kernel picks each item's op-set body from pad2
+        std::vector<PassDev> fp = p.passes;
+        for (size_t k = 0; k < fp.size(); ++k) fp[k].pad2 = pass_ops[k];
+        to_device(&flow_passes, fp, s);
         hip_check(hipMalloc(reinterpret_cast<void **>(&flow_done), flow_items * sizeof(uint32_t)), "hipMalloc(flow stamps)");
         hip_check(hipMalloc(reinterpret_cast<void **>(&flow_ctr), 4 * sizeof(uint32_t)), "hipMalloc(flow counters)");
         flow_ok = true;
@@ -2084,11 +2120,21 @@ static void evolve_pointers_t(const DevicePlan &dp, void *prev_v, void *cur_v, u
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1, kOpsFab>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1, kOpsFcc>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     for (size_t k = 0; k < p.passes.size(); ++k) {
         const PassDev &ps = p.passes[k];
         if (ps.kind == PASS_COLG)
             hipLaunchKernelGGL((pass_kernel<R, 2>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev,
                                cur, j);
+        else if (dp.pass_ops[k] == kOpsFab)
+            hipLaunchKernelGGL((pass_kernel<R, 1, kOpsFab>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A,
+                               ps, prev, cur, j);
+        else if (dp.pass_ops[k] == kOpsFcc)
+            hipLaunchKernelGGL((pass_kernel<R, 1, kOpsFcc>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A,
+                               ps, prev, cur, j);
         else
             hipLaunchKernelGGL((pass_kernel<R, 1>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev,
                                cur, j);
@@ -2465,9 +2511,9 @@ void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end,
         throw std::runtime_error("the dataflow evolution needs LDS-tiled passes and blocks below 2^31 bytes");
     // auto: the dataflow kernel for complex128, whose tiles are half as wide (twice the
     // items per pass, so more passes overlap); complex64 measured faster pass by pass
-    // (32 spin-orbitals / 8 electrons: 58.9 against 68.2 ms per complex128 step, 36.7
-    // against 35.0 ms per complex64 step; 28 spin-orbitals / 10 electrons: 46.3 against
-    // 48.6 and 30.3 against 26.5 ms)
+    // (32 spin-orbitals / 8 electrons: 55.7 against 64.4 ms per complex128 step, 34.3
+    // against 33.4 ms per complex64 step; 28 spin-orbitals / 10 electrons: complex64 29.9
+    // against 25.8 ms)
     const bool flow = execution == EXEC_DATAFLOW || (execution == EXEC_AUTO && dp.dtype == 0);
     if (flow && dp.flow_ok) {
         if (dp.dtype == 0)

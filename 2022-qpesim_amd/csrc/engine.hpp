@@ -32,6 +32,7 @@ struct DevicePlan {
     double *partials = nullptr;   // [max_items] norm partial sums
     size_t lds_bytes = 0;         // largest dynamic LDS of a pass launch
     std::vector<size_t> pass_lds; // dynamic LDS of each pass launch
+    std::vector<uint8_t> pass_ops; // op-type set of each pass's kernel (kOpsAll / kOpsFab / kOpsFcc)
     // dataflow order of a step (build_flow), for plans whose passes are all LDS-tiled
     // and whose blocks are below 2^31 bytes
     bool flow_ok = false;
