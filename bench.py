@@ -343,7 +343,8 @@ def main(argv=None):
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
                            "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
-                           "kernel": "pass_kernel<float, 1> (term-rotation tile pass)",
+                           "kernel": "pass_kernel<float, 1, *> (term-rotation tile passes: the mixed-window, "
+                                     "same-spin and general op-set instantiations, averaged over a step's launches)",
                            "bytes_per_launch": bytes_launch, "launches_per_step": passes_per_step,
                            "avg_launch_us": avg_launch_s * 1e6,
                            "timing": "HIP events on the state's stream around the timed steps"}
