@@ -826,7 +826,11 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     const ItemDev it = A.items[pass.item_begin + item];
     const int nl = (int)it.count;
     const uint32_t C = A.C;
-    const bool col = pass.kind != PASS_ROW;   // COL and COLG lines are columns from the column list
+    // a mixed-window item of the dataflow kernel (FLOW, OPS == kOpsFab) is a COL pass in the
+    // middle of a step with no staged group angles: its body carries none of the other
+    // passes' paths (the dataflow kernel holds three bodies; measured 3% off the batch)
+    constexpr bool kMid = FLOW && OPS == kOpsFab;
+    const bool col = kMid || pass.kind != PASS_ROW;   // COL and COLG lines are columns from the column list
     double2 *gang = reinterpret_cast<double2 *>(smem + kLdsHeader + (size_t)pass.max_lines * C * sizeof(V));
     const int o0 = pass.op_begin, o1 = pass.op_end;
 
@@ -847,7 +851,7 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
                             pair_lists + w * (kMaxLines / 2), pair_counts + w, nl,
                             C * (uint32_t)sizeof(V));
     }
-    for (int i = tid; i < pass.group_count * nl; i += kThreads) {
+    for (int i = tid; !kMid && i < pass.group_count * nl; i += kThreads) {
         int g = i / nl, l = i - g * nl;
         const double *gb = A.grp_b + 2ull * (pass.group_first + g) * C + 2ull * line_rank[l];
         gang[i] = make_double2(gb[0], gb[1]);
@@ -857,7 +861,7 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     if (!gate()) return false;
 
     // ---- load the tile ---------------------------------------------------
-    const bool from_prev = pass.flags & PF_FROM_PREV;
+    const bool from_prev = !kMid && (pass.flags & PF_FROM_PREV);
     const V *src = from_prev ? prev : cur;
     const double sc = (from_prev && A.renorm) ? (FLOW ? coh_load_f64(A.scale + j - 1) : A.scale[j - 1]) : 1.0;
     const uint32_t total = (uint32_t)nl * C;
@@ -884,7 +888,7 @@ __device__ __forceinline__ bool pass_body(const PassArgs &A, const PassDev &pass
     __syncthreads();
 
     // ---- FP64 norm of the tile (last pass of a step) ----------------------
-    if (pass.flags & PF_EMIT_NORM) {
+    if (!kMid && (pass.flags & PF_EMIT_NORM)) {
         KSum acc;
         for (uint32_t idx = tid; idx < total; idx += kThreads) {
             V v = tile[idx];
@@ -2030,7 +2034,10 @@ void DevicePlan::upload(const Plan &p, int dt, hipStream_t s) {
     pass_lds.clear();
     pass_ops.clear();
     for (const auto &ps : p.passes) {
-        bool fab = ps.kind == PASS_COL && ps.op_end > ps.op_begin, fcc = ps.kind != PASS_COLG && ps.op_end > ps.op_begin;
+        // mixed windows: COL passes of FAB ops only, never the first or last of a step, no group angles
+        bool fab = ps.kind == PASS_COL && ps.op_end > ps.op_begin && ps.group_count == 0 &&
+                   !(ps.flags & (PF_FROM_PREV | PF_EMIT_NORM));
+        bool fcc = ps.kind != PASS_COLG && ps.op_end > ps.op_begin;
         for (int o = ps.op_begin; o < ps.op_end; ++o) {
             fab = fab && p.ops[o].type == OP_FAB;
             fcc = fcc && p.ops[o].type == OP_FCC;
