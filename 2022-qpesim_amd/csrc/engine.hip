@@ -2127,15 +2127,17 @@ static void evolve_pointers_t(const DevicePlan &dp, void *prev_v, void *cur_v, u
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 2, kOpsFab>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1, kOpsFab>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<R, 1, kOpsFcc>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dp.lds_bytes);
     for (size_t k = 0; k < p.passes.size(); ++k) {
         const PassDev &ps = p.passes[k];
-        if (ps.kind == PASS_COLG)
-            hipLaunchKernelGGL((pass_kernel<R, 2>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A, ps, prev,
-                               cur, j);
+        if (ps.kind == PASS_COLG)   // windows in global memory: FAB ops only
+            hipLaunchKernelGGL((pass_kernel<R, 2, kOpsFab>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A,
+                               ps, prev, cur, j);
         else if (dp.pass_ops[k] == kOpsFab)
             hipLaunchKernelGGL((pass_kernel<R, 1, kOpsFab>), dim3(ps.item_count), dim3(kThreads), dp.pass_lds[k], s, A,
                                ps, prev, cur, j);
