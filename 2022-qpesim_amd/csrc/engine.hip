@@ -445,6 +445,30 @@ __device__ inline void run_op(const PassArgs &A, const OpDev &d, const typename 
     const uint32_t lbase = __umul24((uint32_t)E.lane, line_bytes);   // the thread's first line (line-local ops)
     if constexpr (OPS == kOpsFcc) {
         if (E.lane >= E.L) return;
+        if (nl <= E.L) {
+            // at most one line per thread (a tile has fewer lines than a slot has threads):
+            // no line loop, and the coefficients of all rows read together (rows past lg
+            // repeat row 0, so every slot index is in range).  A third fewer instructions per
+            // row; the same-spin passes are bound by VALU issue on their busiest SIMD
+            if (E.lane >= nl) return;
+            V cf[kSchedMax];
+#pragma unroll
+            for (int i = 0; i < kSchedMax; ++i) cf[i] = coef[s_slot(E.e[i])];
+            char *tb = reinterpret_cast<char *>(tile) + lbase;
+#pragma unroll
+            for (int i = 0; i < kSchedMax; ++i) {
+                if (i >= d.lg) break;
+                const uint32_t e = E.e[i];
+                if (s_rank1(e) == kSchedPad) continue;
+                V *pu = reinterpret_cast<V *>(tb + s_rank1(e) * (uint32_t)sizeof(V));
+                V *pv = reinterpret_cast<V *>(tb + s_rank2(e) * (uint32_t)sizeof(V));
+                V u = *pu, v = *pv;
+                rotate(u, v, cf[i].x, flip_sign(cf[i].y, ~s_par(e) << 31), entry_dither(z, i));
+                *pu = u;
+                *pv = v;
+            }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < kSchedMax; ++i) {
             if (i >= d.lg) break;
