@@ -2544,9 +2544,9 @@ void evolve(const DevicePlan &dp, void *state, uint64_t j_begin, uint64_t j_end,
         throw std::runtime_error("the dataflow evolution needs LDS-tiled passes and blocks below 2^31 bytes");
     // auto: the dataflow kernel for complex128, whose tiles are half as wide (twice the
     // items per pass, so more passes overlap); complex64 measured faster pass by pass
-    // (32 spin-orbitals / 8 electrons: 55.7 against 64.4 ms per complex128 step, 34.3
-    // against 33.4 ms per complex64 step; 28 spin-orbitals / 10 electrons: complex64 29.9
-    // against 25.8 ms)
+    // (32 spin-orbitals / 8 electrons: 53.4 against 62.9 ms per complex128 step, 32.9
+    // against 32.6 ms per complex64 step; 28 spin-orbitals / 10 electrons: complex128 43.0
+    // against 45.9 ms, complex64 28.1 against 25.8 ms)
     const bool flow = execution == EXEC_DATAFLOW || (execution == EXEC_AUTO && dp.dtype == 0);
     if (flow && dp.flow_ok) {
         if (dp.dtype == 0)
